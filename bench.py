@@ -1,0 +1,304 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X transposition path.
+
+Workload (one "step"): the 16 seeded random-permutation cases of rank 2..6,
+each at 200 MiB of A, in fp32 and in fp64, with beta = 0 (B not read,
+non-temporal stores) and beta = 1 (B read) — 64 out-of-place transpositions
+B[pi(i)] = alpha*A[i] + beta*B[pi(i)], inputs resident in HBM.  All beta = 0
+ops run first, then all beta = 1 ops, so consecutive ops never touch the same
+buffers and no op finds its data in the 256 MB last-level cache.
+
+Metric: GB/s (10^9 B/s) of algorithmic traffic, N*sA + N*sB (+ N*sB when
+beta != 0) (ttc/transposeGenerator.py:665-668), over the whole timed region,
+plus a roofline line for the dominant kernel and the oracle's CPU rate on a
+bounded sample.  `--batch` adds the grouped-launch comparison (512 mixed ops,
+one persistent launch, hipGraph replay) in transposes/s.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-batch]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def kernel_name(info):
+    """rocprof-visible identity of the kernel a plan launches."""
+    ops = {(0, 0): "OpF32<float, float>", (1, 1): "OpDD"}
+    mode = "BETA" if not info["beta_is_zero"] else ("BZ_NT" if info["nt_store"] else "BZ")
+    return "transpose_kernel<%s, %d, %d, %s>" % (ops.get((info["type_a"], info["type_b"]), "?"), info["vec_a"],
+                                                  info["vec_b"], mode)
+
+
+def build_c2(torch, rt, wl, dev):
+    ops = []
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(0)
+    bufs = {}
+    for case in wl.plain_c2():
+        for es, t, tdt in ((4, rt.S, torch.float32), (8, rt.D, torch.float64)):
+            size = case["sizes"][es]
+            lda = case["lda"][es] if case["lda"] else None
+            ldb = case["ldb"][es] if case["ldb"] else None
+            r = len(size)
+            LA = lda or size
+            LB = ldb or [size[case["perm"][q]] for q in range(r)]
+            na = 1
+            for x in LA[:-1]:
+                na *= x
+            na *= size[-1]
+            nb = 1
+            for x in LB[:-1]:
+                nb *= x
+            nb *= size[case["perm"][-1]]
+            A = torch.rand(na, generator=gen, device=dev, dtype=tdt) * 2 - 1
+            B = torch.rand(nb, generator=gen, device=dev, dtype=tdt) * 2 - 1
+            bufs[(case["k"], es)] = (A, B)
+            for bz in (True, False):
+                plan = rt.Plan(t, t, case["perm"], size, lda, ldb, beta_is_zero=bz)
+                info = plan.info()
+                ops.append({"k": case["k"], "es": es, "bz": bz, "plan": plan, "A": A, "B": B,
+                            "alpha": 1.0, "beta": 0.0 if bz else 1.0, "perm": case["perm"], "size": size,
+                            "lda": lda, "ldb": ldb, "bytes": info["bytes_algorithmic"], "info": info,
+                            "kernel": kernel_name(info)})
+    # beta = 0 ops first, then beta = 1 ops
+    ops.sort(key=lambda o: (not o["bz"], o["k"], o["es"]))
+    return ops, bufs
+
+
+def verify_sample(torch, op, nsample=4096):
+    """For a beta = 0, alpha = 1 op: B at sampled output positions equals A
+    at the matching input positions (exact)."""
+    import numpy as np
+    rng = np.random.default_rng(op["k"] * 10 + op["es"])
+    size, perm = op["size"], op["perm"]
+    r = len(size)
+    LA = op["lda"] or size
+    LB = op["ldb"] or [size[perm[q]] for q in range(r)]
+    idx = np.stack([rng.integers(0, s, nsample) for s in size])
+    offA = np.zeros(nsample, np.int64)
+    m = 1
+    for k in range(r):
+        offA += idx[k] * m
+        m *= LA[k]
+    offB = np.zeros(nsample, np.int64)
+    m = 1
+    for q in range(r):
+        offB += idx[perm[q]] * m
+        m *= LB[q]
+    dev = op["A"].device
+    a = op["A"][torch.from_numpy(offA).to(dev)]
+    b = op["B"][torch.from_numpy(offB).to(dev)]
+    return bool(torch.equal(a, b))
+
+
+def run_c2(torch, ops, stream, steps, warmup):
+    for _ in range(warmup):
+        for o in ops:
+            o["plan"].execute(o["A"], o["B"], o["alpha"], o["beta"], stream=stream)
+    torch.cuda.synchronize()
+    ok = all(verify_sample(torch, o) for o in ops if o["bz"])
+    ev = [[(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in ops]
+          for _ in range(steps)]
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    wall0 = time.perf_counter()
+    t0.record(stream)
+    for s in range(steps):
+        for i, o in enumerate(ops):
+            ev[s][i][0].record(stream)
+            o["plan"].execute(o["A"], o["B"], o["alpha"], o["beta"], stream=stream)
+            ev[s][i][1].record(stream)
+    t1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - wall0
+    total_ms = t0.elapsed_time(t1)
+    per_op_ms = [[ev[s][i][0].elapsed_time(ev[s][i][1]) for i in range(len(ops))] for s in range(steps)]
+    return total_ms, wall, per_op_ms, ok
+
+
+def cpu_baseline(budget_s):
+    """The oracle (C++ restatement of TTC's reference loop nest, OpenMP on all
+    host cores) on a bounded sample of the same workload."""
+    import numpy as np
+
+    from oracle import oracle
+    from ttc_amd import workloads as wl
+    nthreads = os.cpu_count() or 1
+    cases = wl.plain_c2()
+    done, tot_bytes, tot_s = [], 0, 0.0
+    trash = np.ones((512 << 20) // 8)
+    t_start = time.perf_counter()
+    for case in [cases[i] for i in (2, 6, 11, 3, 0, 9)]:
+        if time.perf_counter() - t_start > budget_s:
+            break
+        size, perm = case["sizes"][4], case["perm"]
+        n = int(np.prod(size))
+        rng = np.random.default_rng(case["k"])
+        A = rng.uniform(-1, 1, n).astype(np.float32)
+        B = rng.uniform(-1, 1, n).astype(np.float32)
+        for bz in (True, False):
+            best = None
+            for _ in range(2):
+                oracle.trash_cache(trash, nthreads)
+                t = time.perf_counter()
+                oracle.transpose(oracle.S, oracle.S, A, B, 1.0, 0.0 if bz else 1.0, perm, size, nthreads=nthreads)
+                dt = time.perf_counter() - t
+                best = dt if best is None else min(best, dt)
+            tot_bytes += n * 4 * (2 if bz else 3)
+            tot_s += best
+            done.append("k%d/%s" % (case["k"], "b0" if bz else "b1"))
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"value": round(tot_bytes / tot_s / 1e9, 3), "unit": "GB/s", "cores": nthreads, "kind": "port",
+            "sample": "oracle loop nest (OpenMP, %d threads, %s) on fp32 C2 cases %s, min of 2 reps each after a "
+                      "512 MiB cache-trash sweep" % (nthreads, model, ",".join(done))}
+
+
+def run_batch(torch, rt, wl, dev, stream, replays):
+    """512 mixed ops: grouped persistent launch (graph replay) vs the per-op
+    launch loop with the same kernels."""
+    ops = wl.batch_c5()
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(5)
+    tmap = {0: (rt.S, torch.float32), 1: (rt.D, torch.float64), 4: (rt.BF16, torch.bfloat16)}
+    desc = []
+    keep = []
+    for o in ops:
+        t, tdt = tmap[o["type"]]
+        n = 1
+        for x in o["size"]:
+            n *= x
+        A = (torch.rand(n, generator=gen, device=dev, dtype=torch.float32) * 2 - 1).to(tdt)
+        B = (torch.rand(n, generator=gen, device=dev, dtype=torch.float32) * 2 - 1).to(tdt)
+        keep.append((A, B))
+        desc.append({"type_a": t, "type_b": t, "perm": o["perm"], "size": o["size"], "A": A, "B": B,
+                     "alpha": o["alpha"], "beta": o["beta"]})
+    batch = rt.Batch(desc)
+    info = batch.info()
+
+    def timed(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    t_group = timed(lambda: batch.graph_launch(stream, per_op_loop=False), replays)
+    t_group_eager = timed(lambda: batch.execute(stream), max(10, replays // 10))
+    t_loop = timed(lambda: batch.execute_loop(stream), max(10, replays // 10))
+    t_loop_graph = timed(lambda: batch.graph_launch(stream, per_op_loop=True), max(10, replays // 10))
+    n = len(ops)
+    return {"ops": n, "replays": replays, "tiles": info["tiles"], "workgroups": info["workgroups"],
+            "bytes_per_launch": info["bytes"],
+            "grouped_graph_ms": round(t_group, 4), "grouped_eager_ms": round(t_group_eager, 4),
+            "loop_async_ms": round(t_loop, 4), "loop_graph_ms": round(t_loop_graph, 4),
+            "transposes_per_s_grouped": round(n / (t_group * 1e-3), 1),
+            "transposes_per_s_loop": round(n / (t_loop * 1e-3), 1),
+            "speedup_vs_async_loop": round(t_loop / t_group, 3),
+            "speedup_vs_graph_loop": round(t_loop_graph / t_group, 3),
+            "grouped_GBps": round(info["bytes"] / (t_group * 1e-3) / 1e9, 1)}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--batch", dest="batch", action="store_true", default=True)
+    ap.add_argument("--no-batch", dest="batch", action="store_false")
+    ap.add_argument("--batch-replays", type=int, default=200)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only (every box has one MI355X); "
+              "--gpus %d is not supported — the path would shard as independent sub-transpositions "
+              "(see DESIGN.md)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from ttc_amd import runtime as rt
+    from ttc_amd import workloads as wl
+    if not torch.cuda.is_available():
+        print("bench.py: no GPU", file=sys.stderr)
+        sys.exit(1)
+    dev = torch.device("cuda:0")
+    rt.lib()
+    stream = torch.cuda.Stream(device=dev)
+    with torch.cuda.stream(stream):
+        ops, bufs = build_c2(torch, rt, wl, dev)
+    torch.cuda.synchronize()
+    log("c2: %d ops per step" % len(ops))
+    total_ms, wall, per_op, ok = run_c2(torch, ops, stream, args.steps, args.warmup)
+    step_bytes = sum(o["bytes"] for o in ops)
+    value = step_bytes * args.steps / (total_ms * 1e-3) / 1e9
+    # per-kernel aggregation (the roofline line is for the kernel with the most time)
+    groups = {}
+    for s in range(args.steps):
+        for i, o in enumerate(ops):
+            g = groups.setdefault(o["kernel"], {"ms": 0.0, "bytes": 0, "launches": 0})
+            g["ms"] += per_op[s][i]
+            g["bytes"] += o["bytes"]
+            g["launches"] += 1
+    dom = max(groups, key=lambda k: groups[k]["ms"])
+    gd = groups[dom]
+    achieved = gd["bytes"] / (gd["ms"] * 1e-3) / 1e9
+    for i, o in enumerate(ops):
+        ms = min(per_op[s][i] for s in range(args.steps))
+        log("  k=%2d %s %s rank=%d tile=%s vec=%d/%d %8.1f GB/s  %.3f ms  %s" % (
+            o["k"], "f32" if o["es"] == 4 else "f64", "b0" if o["bz"] else "b1", o["info"]["rank"],
+            o["info"]["tile"], o["info"]["vec_a"], o["info"]["vec_b"], o["bytes"] / (ms * 1e-3) / 1e9, ms,
+            o["kernel"]))
+    for k, g in sorted(groups.items(), key=lambda kv: -kv[1]["ms"]):
+        log("  kernel %-48s launches=%4d avg=%.4f ms  %.1f GB/s" % (
+            k, g["launches"], g["ms"] / g["launches"], g["bytes"] / (g["ms"] * 1e-3) / 1e9))
+    out = {
+        "metric": "GB/s (r+w) & % of HBM peak per transpose; batched transposes/sec vs plain",
+        "value": round(value, 2), "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(total_ms / args.steps, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32,f64", "data": "synthetic",
+        "config": {"workload": "16 seeded random perms rank 2-6, 200 MiB A each, fp32+fp64, beta 0 and 1 "
+                               "(64 transposes per step)", "bytes_per_step": step_bytes,
+                   "pct_of_hbm_peak": round(100.0 * value / HBM_PEAK_GBS, 2), "verified_sample": ok},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "avg_launch_ms": round(gd["ms"] / gd["launches"], 5),
+                     "bytes_per_launch": gd["bytes"] // gd["launches"]},
+    }
+    if args.batch:
+        del bufs, ops
+        torch.cuda.empty_cache()
+        try:
+            out["batched"] = run_batch(torch, rt, wl, dev, stream, args.batch_replays)
+        except Exception as e:  # reported, not hidden
+            out["batched"] = {"error": str(e)}
+    out["cpu_baseline"] = None if args.no_cpu else cpu_baseline(args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

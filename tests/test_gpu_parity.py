@@ -1,0 +1,201 @@
+"""GPU parity: the HIP path (through the C ABI) against the oracle, bit for
+bit, on the reference's own test sweep, the benchmark shapes and the edge
+cases (padding, partial tiles, size-1 and empty dimensions, misaligned
+pointers, every vector width and store mode)."""
+import numpy as np
+import pytest
+
+from common import (BF16, C, CODE, D, F16, S, Z, extents, first_diff, from_device, make_case, oracle_run,
+                    same_bits, to_device)
+from test_oracle import ttc_test_cases
+
+pytestmark = pytest.mark.gpu
+
+
+def run_hip(device, tA, tB, A, B, alpha, beta, perm, size, lda=None, ldb=None, beta_is_zero=None, config=None,
+            offset_a=0, offset_b=0):
+    import torch
+    from ttc_amd import runtime as rt
+    if beta_is_zero is None:
+        beta_is_zero = beta == 0
+    dA = to_device(np.concatenate([np.zeros(offset_a, A.dtype), A]) if offset_a else A, device)
+    dB = to_device(np.concatenate([np.zeros(offset_b, B.dtype), B]) if offset_b else B, device)
+    plan = rt.Plan(tA, tB, perm, size, lda, ldb, beta_is_zero, config)
+    plan.execute(dA.data_ptr() + offset_a * A.itemsize, dB.data_ptr() + offset_b * B.itemsize, alpha, beta)
+    torch.cuda.synchronize()
+    out = dB.cpu().numpy().view(B.dtype)
+    return out[offset_b:].copy(), plan.info()
+
+
+def check(device, tA, tB, perm, size, alpha, beta, lda=None, ldb=None, seed=0, dist="uniform", config=None,
+          beta_is_zero=None, offset_a=0, offset_b=0):
+    A, B = make_case(tA, tB, perm, size, lda, ldb, seed=seed, dist=dist)
+    want = oracle_run(tA, tB, A, B, alpha, beta, perm, size, lda, ldb, beta_is_zero)
+    got, info = run_hip(device, tA, tB, A, B, alpha, beta, perm, size, lda, ldb, beta_is_zero, config,
+                        offset_a, offset_b)
+    if not same_bits(got, want):
+        idx, n = first_diff(got, want)
+        raise AssertionError("mismatch %s->%s perm=%s size=%s lda=%s ldb=%s a=%s b=%s cfg=%s: %d elements differ, "
+                             "first %s; got %s want %s; plan %s" % (tA, tB, perm, size, lda, ldb, alpha, beta, config,
+                                                                    n, idx, got[idx], want[idx], info))
+    return info
+
+
+@pytest.mark.parametrize("code", ["d", "s", "c", "z", "sd", "ds", "cz", "zc"])
+def test_reference_sweep(gpu, code):
+    """test-framework/ttc_test.py's matrix (perms, sizes, alpha 1.1 / 1.0,
+    beta 1, -1, 1.2, 0), all bit-exact against the oracle."""
+    tA, tB = CODE[code]
+    n = 0
+    for i, (perm, size, c, alpha, beta) in enumerate(ttc_test_cases()):
+        if c != code:
+            continue
+        check(gpu, tA, tB, perm, size, alpha, beta, seed=i, dist="normal")
+        n += 1
+    assert n > 70
+
+
+def test_c1_full_size(gpu):
+    """fp32 perm (1,0,2), size (1000,768,16), alpha = beta = 1, seed 0."""
+    check(gpu, S, S, [1, 0, 2], [1000, 768, 16], 1.0, 1.0, seed=0)
+    check(gpu, S, S, [1, 0, 2], [1000, 768, 16], 1.0, 0.0, seed=0)
+
+
+RANDOM_SHAPES = [
+    ([5, 3, 0, 4, 1, 2], [24, 20, 18, 22, 19, 17]),
+    ([1, 0], [1531, 977]), ([2, 0, 1], [97, 130, 81]), ([0, 2, 1], [33, 257, 65]),
+    ([3, 1, 0, 2], [41, 19, 37, 23]), ([0, 3, 2, 1], [8, 33, 29, 31]), ([4, 2, 0, 3, 1], [13, 17, 11, 9, 21]),
+    ([2, 1, 0], [1, 600, 700]), ([1, 0, 2], [3, 5000, 40]), ([0, 1], [1000, 1000]), ([1, 0], [4096, 8]),
+    ([1, 0], [8, 4096]), ([3, 2, 1, 0], [9, 10, 11, 12]), ([0, 2, 1, 3], [128, 512, 32, 2]),
+]
+
+
+@pytest.mark.parametrize("code", ["s", "d", "c", "z", "sd", "ds", "cz", "zc", "b", "h", "sb", "sh", "bs", "hs"])
+def test_random_shapes(gpu, code):
+    tA, tB = CODE[code]
+    for i, (perm, size) in enumerate(RANDOM_SHAPES):
+        for beta in (0.0, 0.7):
+            check(gpu, tA, tB, perm, size, 1.3, beta, seed=100 + i, dist="outliers")
+
+
+def test_padded_subtensor(gpu):
+    rng = np.random.default_rng(15)
+    for i, (perm, size) in enumerate(RANDOM_SHAPES[:8]):
+        lda = [s + int(rng.integers(1, 32)) for s in size]
+        ldb = [size[perm[q]] + int(rng.integers(1, 32)) for q in range(len(size))]
+        for code in ("s", "d", "z", "ds"):
+            tA, tB = CODE[code]
+            check(gpu, tA, tB, perm, size, 1.0, 1.0, lda, ldb, seed=i)
+            check(gpu, tA, tB, perm, size, 1.1, 0.0, lda, None, seed=i)
+            check(gpu, tA, tB, perm, size, 0.9, -1.0, None, ldb, seed=i)
+
+
+def test_every_vector_width_and_mode(gpu):
+    from ttc_amd import runtime as rt
+    from ttc_amd.gen_instances import vec_widths
+    perm, size = [2, 0, 1], [64, 48, 80]
+    for code in ("s", "d", "b", "c", "sb", "ds"):
+        tA, tB = CODE[code]
+        for va in vec_widths(rt.DTYPE_BYTES[tA]):
+            for vb in vec_widths(rt.DTYPE_BYTES[tB]):
+                for nt in (1, 2):
+                    for order in (1, 2):
+                        cfg = dict(vec_a=va, vec_b=vb, nt_store=nt, grid_order=order, wg_per_cu=1)
+                        for beta in (0.0, 1.5):
+                            check(gpu, tA, tB, perm, size, 1.25, beta, config=cfg, seed=va * 10 + vb)
+
+
+def test_explicit_tiles_with_two_partial_dims(gpu):
+    perm, size = [1, 0, 2], [100, 90, 7]
+    for tile in ([32, 32, 1], [36, 28, 3], [100, 7, 2], [4, 90, 1], [20, 50, 7]):
+        check(gpu, S, S, perm, size, 1.0, 1.0, config=dict(tile=tile, vec_a=4 if tile[0] % 4 == 0 else 1,
+                                                              vec_b=1 if tile[1] % 2 else 2))
+
+
+def test_misaligned_pointers_use_scalar_variant(gpu):
+    for off in (1, 3):
+        check(gpu, S, S, [1, 0], [256, 192], 1.0, 1.0, offset_a=off, offset_b=0)
+        check(gpu, S, S, [1, 0], [256, 192], 1.0, 0.0, offset_a=0, offset_b=off)
+        check(gpu, BF16, BF16, [1, 0], [256, 192], 1.0, 0.0, offset_a=off, offset_b=off)
+
+
+def test_edge_shapes(gpu):
+    check(gpu, S, S, [0], [1], 2.0, 0.0)
+    check(gpu, S, S, [0], [12345], 2.0, 1.0)
+    check(gpu, D, D, [1, 0], [1, 1], 2.0, 1.0)
+    check(gpu, S, S, [2, 1, 0], [1, 1, 5], 1.0, 0.0)
+    check(gpu, Z, Z, [3, 2, 1, 0], [2, 1, 3, 1], 1.0, 1.0)
+    check(gpu, S, S, [0, 1, 2, 3, 4, 5], [2, 3, 4, 5, 6, 7], 1.0, 0.0)  # identity -> fused rank 1
+    # rank 16 before fusion, several size-1 dimensions
+    perm = [15 - i for i in range(16)]
+    size = [2, 1, 2, 1, 2, 1, 2, 1, 1, 2, 1, 1, 1, 2, 1, 1]
+    check(gpu, S, S, perm, size, 1.0, 1.0)
+
+
+def test_empty_tensor_is_a_no_op(gpu):
+    import torch
+    from ttc_amd import runtime as rt
+    plan = rt.Plan(S, S, [1, 0], [0, 5])
+    plan.execute(0, 0, 1.0, 0.0)
+    torch.cuda.synchronize()
+
+
+def test_invalid_pointer_alignment_is_an_error(gpu):
+    import torch
+    from ttc_amd import runtime as rt
+    buf = torch.zeros(4096, dtype=torch.uint8, device=gpu)
+    plan = rt.Plan(S, S, [1, 0], [16, 16])
+    with pytest.raises(rt.TTCError):
+        plan.execute(buf.data_ptr() + 1, buf.data_ptr() + 2048, 1.0, 0.0)
+
+
+def test_denormals_signed_zeros_and_infinities_are_copied_exactly(gpu):
+    """alpha = 1, beta = 0 copies are bit copies for all non-NaN values."""
+    import torch
+    from ttc_amd import runtime as rt
+    vals = np.array([0.0, -0.0, 1e-45, -1e-45, 1.17e-38, 1e-40, np.inf, -np.inf, 3.4e38, -2.5], np.float32)
+    A = np.resize(vals, 64 * 48).astype(np.float32)
+    B = np.zeros_like(A)
+    want = oracle_run(S, S, A, B, 1.0, 0.0, [1, 0], [64, 48])
+    got, _ = run_hip(gpu, S, S, A, B, 1.0, 0.0, [1, 0], [64, 48])
+    assert same_bits(got, want)
+    At = A.reshape(64, 48, order="F").T.reshape(-1, order="F")
+    assert same_bits(got, At)
+    # double precision and beta != 0 with denormal products
+    Ad = np.resize(np.array([5e-324, -5e-324, 2.2e-308, 0.0, -0.0], np.float64), 64 * 48)
+    Bd = np.resize(np.array([1e-310, -0.0, 1.0], np.float64), 64 * 48)
+    want = oracle_run(D, D, Ad, Bd, 0.5, 0.5, [1, 0], [64, 48])
+    got, _ = run_hip(gpu, D, D, Ad, Bd, 0.5, 0.5, [1, 0], [64, 48])
+    assert same_bits(got, want)
+
+
+def test_legacy_drop_in_entry(gpu):
+    """ttc_legacy_transpose: synchronous, reference wrapper semantics."""
+    import torch
+    from ttc_amd import runtime as rt
+    perm, size = [1, 0, 2], [100, 76, 16]
+    A, B = make_case(S, S, perm, size, seed=9)
+    want = oracle_run(S, S, A, B, 1.0, 1.0, perm, size, lda=size, ldb=[76, 100, 16])
+    dA, dB = to_device(A, gpu), to_device(B, gpu)
+    rt.legacy_transpose(S, S, dA, dB, 1.0, 1.0, perm, size, lda=size, ldb=[76, 100, 16],
+                        name="sCuTranspose_102_100x76x16")
+    got = from_device(dB, B)
+    assert same_bits(got, want)
+
+
+def test_concurrent_streams_share_a_plan(gpu):
+    import torch
+    from ttc_amd import runtime as rt
+    perm, size = [2, 1, 0], [40, 50, 60]
+    plan = rt.Plan(D, D, perm, size, beta_is_zero=False)
+    cases = []
+    streams = [torch.cuda.Stream() for _ in range(3)]
+    for i, st in enumerate(streams):
+        A, B = make_case(D, D, perm, size, seed=50 + i)
+        dA, dB = to_device(A, gpu), to_device(B, gpu)
+        torch.cuda.synchronize()
+        plan.execute(dA, dB, 1.5, 0.5, stream=st)
+        cases.append((A, B, dA, dB))
+    torch.cuda.synchronize()
+    for A, B, dA, dB in cases:
+        assert same_bits(from_device(dB, B), oracle_run(D, D, A, B, 1.5, 0.5, perm, size))

@@ -1,0 +1,460 @@
+// Device code of libttc_hip.so: one LDS-staged tile kernel family for every
+// transposition shape on gfx950 (CDNA4, wave64).
+//
+// What the reference generates per shape (ttc/CUDAtranspose.py) — a 32x32
+// LDS tile for perm[0] != 0 (289-323), remainder tiles (326-388), a strided
+// row loop for perm[0] == 0 (179-243) and a plain copy for the identity
+// (781-799) — is one kernel here.  A work-group stages a tile that covers
+// whole runs of A's leading dimensions on the read side and of B's leading
+// dimensions on the write side, so both the HBM read and the HBM write are
+// contiguous runs, whatever the rank:
+//   * read phase: lane-consecutive vectors walk the tile in A order
+//     (coalesced loads), alpha is applied in A's type (as the reference does
+//     before its shared-memory store, ttc/CUDAtranspose.py:907), the values
+//     land in LDS at their A-order position;
+//   * write phase: lane-consecutive vectors walk the tile in B order, gather
+//     from LDS, apply the beta epilogue in B's type and store coalesced
+//     (non-temporal when beta == 0: B is not read then).
+// Every thread precomputes, once per kernel, the tile-relative A offset, B
+// offset and LDS index of each vector it owns; a tile then costs one 64-bit
+// base per side.  Work-groups are persistent and walk the tile grid.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/ttc_hip.h"
+#include "ttc_params.h"
+
+namespace ttc {
+
+// ----------------------------------------------------------------- types
+struct alignas(8) c32 { float re, im; };
+struct alignas(16) c64 { double re, im; };
+struct bf16s { uint16_t u; };
+struct f16s { uint16_t u; };
+struct e4m3s { uint8_t u; };
+struct e5m2s { uint8_t u; };
+
+__device__ __forceinline__ uint32_t fdiv(uint32_t n, const FastDiv& f) { return (__umulhi(n, f.m) + n) >> f.s; }
+
+// ------------------------------------------------------------ conversions
+__device__ __forceinline__ float to_f32(float a) { return a; }
+__device__ __forceinline__ float to_f32(bf16s a) { return __uint_as_float((uint32_t)a.u << 16); }
+__device__ __forceinline__ float to_f32(f16s a) {
+  _Float16 h;
+  __builtin_memcpy(&h, &a.u, 2);
+  return (float)h;
+}
+
+// binary32 -> bfloat16, round to nearest even; NaN stays NaN (quiet bit set).
+__device__ __forceinline__ bf16s to_bf16(float x) {
+  uint32_t u = __float_as_uint(x);
+  bf16s r;
+  if ((u & 0x7fffffffu) > 0x7f800000u) {
+    r.u = (uint16_t)((u >> 16) | 0x40u);
+  } else {
+    r.u = (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+  }
+  return r;
+}
+// binary32 -> binary16 with the hardware conversion (round to nearest even,
+// gradual underflow, overflow to Inf).
+__device__ __forceinline__ f16s to_f16(float x) {
+  _Float16 h = (_Float16)x;
+  f16s r;
+  __builtin_memcpy(&r.u, &h, 2);
+  return r;
+}
+
+// binary32 -> OCP fp8 e4m3fn, round to nearest even, saturating to +-448,
+// NaN -> 0x7f.  Normal range: round the binary32 mantissa to 3 bits in the
+// integer domain and rebias the exponent (127 -> 7); subnormal range
+// (|x| < 2^-6): the value is m * 2^-9 with m = rne(|x| * 2^9) (exact scaling).
+__device__ __forceinline__ uint8_t to_e4m3(float x) {
+  const uint32_t u = __float_as_uint(x);
+  const uint32_t sign = (u >> 24) & 0x80u;
+  const uint32_t a = u & 0x7fffffffu;
+  if (a > 0x7f800000u) return (uint8_t)(sign | 0x7fu);
+  if (a >= 0x43e00000u) return (uint8_t)(sign | 0x7eu);  // >= 448 (incl. Inf): saturate
+  if (a >= 0x3c800000u) {                                  // >= 2^-6: normal
+    uint32_t r = (a + 0x7ffffu + ((a >> 20) & 1u)) >> 20;
+    r -= (120u << 3);
+    return (uint8_t)(sign | r);
+  }
+  const float m = __builtin_rintf(__uint_as_float(a) * 512.0f);
+  return (uint8_t)(sign | (uint32_t)m);
+}
+// binary32 -> OCP fp8 e5m2, round to nearest even, saturating to +-57344,
+// NaN -> 0x7f.  Exponent rebias 127 -> 15; subnormal quantum 2^-16.
+__device__ __forceinline__ uint8_t to_e5m2(float x) {
+  const uint32_t u = __float_as_uint(x);
+  const uint32_t sign = (u >> 24) & 0x80u;
+  const uint32_t a = u & 0x7fffffffu;
+  if (a > 0x7f800000u) return (uint8_t)(sign | 0x7fu);
+  if (a >= 0x47600000u) return (uint8_t)(sign | 0x7bu);  // >= 57344 (incl. Inf): saturate
+  if (a >= 0x38800000u) {                                  // >= 2^-14: normal
+    uint32_t r = (a + 0xfffffu + ((a >> 21) & 1u)) >> 21;
+    r -= (112u << 2);
+    return (uint8_t)(sign | r);
+  }
+  const float m = __builtin_rintf(__uint_as_float(a) * 65536.0f);
+  return (uint8_t)(sign | (uint32_t)m);
+}
+
+template <class T> __device__ __forceinline__ T from_f32(float x);
+template <> __device__ __forceinline__ float from_f32<float>(float x) { return x; }
+template <> __device__ __forceinline__ bf16s from_f32<bf16s>(float x) { return to_bf16(x); }
+template <> __device__ __forceinline__ f16s from_f32<f16s>(float x) { return to_f16(x); }
+
+// ---------------------------------------------------------------- epilogues
+// Each Op fixes: the LDS type L (alpha * a in A's type), the real types of
+// alpha (AL) and beta (BE), prod() applied in the read phase and out()/outb()
+// applied in the write phase.  The expressions are the C expressions of the
+// reference's generated code with its declared types (ttc/transpose.py:
+// 126-138 and 402-407; GPU: ttc/CUDAtranspose.py:801-961), evaluated without
+// contraction (the library is built with -ffp-contract=off).
+template <class TA_, class TB_> struct OpF32 {  // A, B in {float, bf16, f16}: arithmetic in fp32
+  using TA = TA_; using TB = TB_; using L = float; using AL = float; using BE = float;
+  __device__ static L prod(AL al, TA a) { return al * to_f32(a); }
+  __device__ static TB out(L x) { return from_f32<TB>(x); }
+  __device__ static TB outb(L x, BE be, TB b) { const float y = be * to_f32(b); return from_f32<TB>(x + y); }
+};
+struct OpDD {
+  using TA = double; using TB = double; using L = double; using AL = double; using BE = double;
+  __device__ static L prod(AL al, TA a) { return al * a; }
+  __device__ static TB out(L x) { return x; }
+  __device__ static TB outb(L x, BE be, TB b) { const double y = be * b; return x + y; }
+};
+struct OpSD {  // A float, B double: fl64((double)fl32(al*a) + fl64(be*b))
+  using TA = float; using TB = double; using L = float; using AL = float; using BE = double;
+  __device__ static L prod(AL al, TA a) { return al * a; }
+  __device__ static TB out(L x) { return (double)x; }
+  __device__ static TB outb(L x, BE be, TB b) { const double y = be * b; return (double)x + y; }
+};
+struct OpDS {  // A double, B float: fl32(fl64(fl64(al*a) + (double)fl32(be*b)))
+  using TA = double; using TB = float; using L = double; using AL = double; using BE = float;
+  __device__ static L prod(AL al, TA a) { return al * a; }
+  __device__ static TB out(L x) { return (float)x; }
+  __device__ static TB outb(L x, BE be, TB b) { const float y = be * b; return (float)(x + (double)y); }
+};
+// Complex: real alpha/beta scale each component (g++ "real * complex" in the
+// generated CPU code), so the complex ops are the real ops per component.
+template <class R, class CA, class CB, class CL> struct OpCplx {
+  using TA = CA; using TB = CB; using L = CL; using AL = typename R::AL; using BE = typename R::BE;
+  __device__ static L prod(AL al, TA a) { L r; r.re = R::prod(al, a.re); r.im = R::prod(al, a.im); return r; }
+  __device__ static TB out(L x) { TB r; r.re = R::out(x.re); r.im = R::out(x.im); return r; }
+  __device__ static TB outb(L x, BE be, TB b) {
+    TB r; r.re = R::outb(x.re, be, b.re); r.im = R::outb(x.im, be, b.im); return r;
+  }
+};
+using OpCC = OpCplx<OpF32<float, float>, c32, c32, c32>;
+using OpZZ = OpCplx<OpDD, c64, c64, c64>;
+using OpCZ = OpCplx<OpSD, c32, c64, c32>;
+using OpZC = OpCplx<OpDS, c64, c32, c64>;
+
+// fp8 outputs: q = sat_rne(fl32(fl32(al*a) * scale)).
+template <class TA_, class TB_> struct OpF8 {
+  using TA = TA_; using TB = TB_; using L = float; using AL = float; using BE = float;
+  __device__ static L prod(AL al, TA a) { return al * to_f32(a); }
+  __device__ static TB outs(L x, float scale) {
+    TB r;
+    const float y = x * scale;
+    if constexpr (__is_same(TB, e4m3s)) r.u = to_e4m3(y); else r.u = to_e5m2(y);
+    return r;
+  }
+};
+
+template <class OP> struct is_fp8 { static constexpr bool value = false; };
+template <class A, class B> struct is_fp8<OpF8<A, B>> { static constexpr bool value = true; };
+
+// ----------------------------------------------------------- vector access
+template <int BYTES> struct Raw;
+template <> struct Raw<1> { using T = uint8_t; };
+template <> struct Raw<2> { using T = uint16_t; };
+template <> struct Raw<4> { using T = uint32_t; };
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+template <> struct Raw<8> { using T = u32x2; };
+template <> struct Raw<16> { using T = u32x4; };
+
+template <class T, int N> __device__ __forceinline__ void vload(T (&dst)[N], const T* p) {
+  using R = typename Raw<sizeof(T) * N>::T;
+  const R r = *reinterpret_cast<const R*>(p);
+  __builtin_memcpy(&dst[0], &r, sizeof(R));
+}
+template <class T, int N, bool NT> __device__ __forceinline__ void vstore(T* p, const T (&src)[N]) {
+  using R = typename Raw<sizeof(T) * N>::T;
+  R r;
+  __builtin_memcpy(&r, &src[0], sizeof(R));
+  if constexpr (NT) __builtin_nontemporal_store(r, reinterpret_cast<R*>(p));
+  else *reinterpret_cast<R*>(p) = r;
+}
+
+// ------------------------------------------------------------- the tile
+// VA / VB: elements per lane vector on the read / write side.  Vectors never
+// cross a tile row: the planner guarantees t[0] % VA == 0 and
+// t[perm[0]] % VB == 0 (and matching strides / extents for alignment).
+template <class OP, int VA, int VB, bool BETA, bool NT>
+struct TileWorker {
+  using TA = typename OP::TA;
+  using TB = typename OP::TB;
+  using L = typename OP::L;
+  static constexpr int NSA = kMaxElemsPerThread / VA;
+  static constexpr int NSB = kMaxElemsPerThread / VB;
+  static constexpr bool FP8 = is_fp8<OP>::value;
+
+  int32_t offA[NSA];
+  int32_t ldsA[NSA];
+  uint32_t crdA[NSA];
+  int32_t offB[NSB];
+  int32_t ldsB[NSB];
+  uint32_t crdB[NSB];
+  int nA, nB;  // slots this thread owns (the last may be partially used)
+
+  // Per-thread, once per launch (or per op in a grouped launch): decompose
+  // this thread's vector positions inside a tile.
+  __device__ __forceinline__ void setup(const TileParams& p, int tid) {
+    const int nvA = p.V / VA;
+    const int nvB = p.V / VB;
+    nA = (nvA - tid + kThreads - 1) / kThreads;
+    nB = (nvB - tid + kThreads - 1) / kThreads;
+#pragma unroll
+    for (int s = 0; s < NSA; ++s) {
+      uint32_t l = (uint32_t)(s * kThreads + tid) * VA;
+      int32_t oa = 0, ol = 0;
+      uint32_t c = 0;
+#pragma unroll
+      for (int k = 0; k < kMaxDim; ++k) {
+        if (k < p.rank) {
+          const uint32_t q = fdiv(l, p.tdiv[k]);
+          const uint32_t j = l - q * (uint32_t)p.t[k];
+          l = q;
+          oa += (int32_t)j * p.sa[k];
+          ol += (int32_t)j * p.ls[k];
+          if (k == p.split0) c |= j;
+          if (k == p.split1) c |= j << 16;
+        }
+      }
+      offA[s] = oa; ldsA[s] = ol; crdA[s] = c;
+    }
+#pragma unroll
+    for (int s = 0; s < NSB; ++s) {
+      uint32_t l = (uint32_t)(s * kThreads + tid) * VB;
+      int32_t ob = 0, ol = 0;
+      uint32_t c = 0;
+#pragma unroll
+      for (int q = 0; q < kMaxDim; ++q) {
+        if (q < p.rank) {
+          const int k = p.perm[q];
+          const uint32_t qq = fdiv(l, p.tdiv[k]);
+          const uint32_t j = l - qq * (uint32_t)p.t[k];
+          l = qq;
+          ob += (int32_t)j * p.sb[k];
+          ol += (int32_t)j * p.ls[k];
+          if (k == p.split0) c |= j;
+          if (k == p.split1) c |= j << 16;
+        }
+      }
+      offB[s] = ob; ldsB[s] = ol; crdB[s] = c;
+    }
+  }
+
+  // One tile.  lim0/lim1: extent of this tile along the split dims.
+  __device__ __forceinline__ void tile(const TileParams& p, const TA* __restrict__ A, TB* __restrict__ B,
+                                       L* __restrict__ lds, uint32_t tile_id, typename OP::AL alpha,
+                                       typename OP::BE beta, float scale, float& amax) {
+    uint32_t rem = tile_id;
+    int64_t bA = 0, bB = 0;
+    uint32_t lim0 = 0xffffu, lim1 = 0xffffu;
+#pragma unroll
+    for (int k = 0; k < kMaxDim; ++k) {
+      if (k < p.rank) {
+        const uint32_t q = fdiv(rem, p.gdiv[k]);
+        const uint32_t c = rem - q * p.gdiv[k].d;
+        rem = q;
+        bA += (int64_t)c * p.tsa[k];
+        bB += (int64_t)c * p.tsb[k];
+        const uint32_t left = (uint32_t)p.n[k] - c * (uint32_t)p.t[k];
+        if (k == p.split0) lim0 = left;
+        if (k == p.split1) lim1 = left;
+      }
+    }
+    const bool full = lim0 >= (uint32_t)p.tsplit0 && lim1 >= (uint32_t)p.tsplit1;
+    const TA* a = A + bA;
+    TB* b = B + bB;
+
+    // read phase: global -> registers -> (alpha) -> LDS
+    TA va[NSA][VA];
+#pragma unroll
+    for (int s = 0; s < NSA; ++s) {
+      const bool ok = s < nA && (full || ((crdA[s] & 0xffffu) < lim0 && (crdA[s] >> 16) < lim1));
+      if (ok) vload<TA, VA>(va[s], a + offA[s]);
+    }
+#pragma unroll
+    for (int s = 0; s < NSA; ++s) {
+      const bool ok = s < nA && (full || ((crdA[s] & 0xffffu) < lim0 && (crdA[s] >> 16) < lim1));
+      if (ok) {
+#pragma unroll
+        for (int v = 0; v < VA; ++v) {
+          const L x = OP::prod(alpha, va[s][v]);
+          if constexpr (FP8) amax = fmaxf(amax, fabsf(x));
+          lds[ldsA[s] + v] = x;
+        }
+      }
+    }
+    __syncthreads();
+    // write phase: LDS -> (beta epilogue) -> global
+    const int S = p.lds_stride_b;
+#pragma unroll
+    for (int s = 0; s < NSB; ++s) {
+      const bool ok = s < nB && (full || ((crdB[s] & 0xffffu) < lim0 && (crdB[s] >> 16) < lim1));
+      if (ok) {
+        TB o[VB];
+        if constexpr (FP8) {
+#pragma unroll
+          for (int v = 0; v < VB; ++v) o[v] = OP::outs(lds[ldsB[s] + v * S], scale);
+        } else if constexpr (BETA) {
+          TB ob[VB];
+          vload<TB, VB>(ob, b + offB[s]);
+#pragma unroll
+          for (int v = 0; v < VB; ++v) o[v] = OP::outb(lds[ldsB[s] + v * S], beta, ob[v]);
+        } else {
+#pragma unroll
+          for (int v = 0; v < VB; ++v) o[v] = OP::out(lds[ldsB[s] + v * S]);
+        }
+        vstore<TB, VB, NT>(b + offB[s], o);
+      }
+    }
+    __syncthreads();
+  }
+};
+
+// Tile index walk of work-group `wg` out of `nwg`: strided (tile = wg + i*nwg)
+// or contiguous ranges.
+struct TileRange {
+  uint32_t first, end, step;
+};
+__device__ __forceinline__ TileRange tile_range(uint32_t ntiles, uint32_t wg, uint32_t nwg, int order) {
+  TileRange r;
+  if (order == 0) {
+    r.first = wg; r.end = ntiles; r.step = nwg;
+  } else {
+    const uint32_t per = ntiles / nwg, extra = ntiles % nwg;
+    r.first = wg * per + (wg < extra ? wg : extra);
+    r.end = r.first + per + (wg < extra ? 1u : 0u);
+    r.step = 1;
+  }
+  return r;
+}
+
+// Block-wide max of a non-negative float, then one atomicMax on its bits
+// (non-negative IEEE floats order like their unsigned bit patterns).
+__device__ __forceinline__ void block_amax_commit(float v, float* out) {
+  __shared__ float red[kThreads / 64];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) red[wave] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float m = red[0];
+#pragma unroll
+    for (int w = 1; w < kThreads / 64; ++w) m = fmaxf(m, red[w]);
+    atomicMax(reinterpret_cast<unsigned int*>(out), __float_as_uint(m));
+  }
+}
+
+// --------------------------------------------------------------- kernels
+template <class OP, int VA, int VB, bool BETA, bool NT>
+__global__ void __launch_bounds__(kThreads) transpose_kernel(const TileParams p, const typename OP::TA* __restrict__ A,
+                                                             typename OP::TB* __restrict__ B, double alpha,
+                                                             double beta) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  using L = typename OP::L;
+  L* lds = reinterpret_cast<L*>(lds_raw);
+  TileWorker<OP, VA, VB, BETA, NT> w;
+  w.setup(p, threadIdx.x);
+  const TileRange r = tile_range(p.ntiles, blockIdx.x, gridDim.x, p.grid_order);
+  float amax = 0.0f;
+  const typename OP::AL al = (typename OP::AL)alpha;
+  const typename OP::BE be = (typename OP::BE)beta;
+  for (uint32_t t = r.first; t < r.end; t += r.step) w.tile(p, A, B, lds, t, al, be, 1.0f, amax);
+}
+
+// fp8 cast-transpose.  The scale comes from *scale_in (delayed scaling),
+// from fp8_max / *amax_in (current scaling, amax_in produced by a previous
+// pass), or from the host value.
+template <class OP, int VA, int VB>
+__global__ void __launch_bounds__(kThreads) cast_fp8_kernel(const TileParams p, const typename OP::TA* __restrict__ A,
+                                                            typename OP::TB* __restrict__ B, float alpha,
+                                                            const float* __restrict__ scale_in,
+                                                            const float* __restrict__ amax_in, float fp8_max,
+                                                            float scale_host, float* __restrict__ amax_out,
+                                                            float* __restrict__ scale_inv_out) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  float* lds = reinterpret_cast<float*>(lds_raw);
+  float scale = scale_host;
+  if (scale_in) {
+    scale = *scale_in;
+  } else if (amax_in) {
+    const float m = *amax_in;
+    scale = (m > 0.0f && m <= 3.402823466e38f) ? fp8_max / m : 1.0f;
+  }
+  if (scale_inv_out && blockIdx.x == 0 && threadIdx.x == 0) *scale_inv_out = 1.0f / scale;
+  TileWorker<OP, VA, VB, false, true> w;
+  w.setup(p, threadIdx.x);
+  const TileRange r = tile_range(p.ntiles, blockIdx.x, gridDim.x, p.grid_order);
+  float amax = 0.0f;
+  for (uint32_t t = r.first; t < r.end; t += r.step) w.tile(p, A, B, lds, t, alpha, 0.0f, scale, amax);
+  if (amax_out) block_amax_commit(amax, amax_out);
+}
+
+
+// amax of alpha*A, current scaling pass 1.  Dense A: 16-byte vector loads
+// over the flat array.  Strided A (padded lda): element e of the A-order walk
+// is (i0 = e mod n0, row = e div n0), row decomposed over dims 1..rank-1.
+struct AmaxParams {
+  int32_t rank;
+  int32_t n0;
+  int32_t dense;
+  FastDiv dn0;
+  FastDiv dn[kMaxDim];
+  int64_t sa[kMaxDim];
+};
+template <class TA, int VEC>
+__global__ void __launch_bounds__(kThreads) amax_kernel(const AmaxParams q, const TA* __restrict__ A, float alpha,
+                                                        float* __restrict__ amax_out, uint64_t N) {
+  float m = 0.0f;
+  const uint64_t stride = (uint64_t)gridDim.x * kThreads;
+  if (q.dense) {
+    for (uint64_t e = ((uint64_t)blockIdx.x * kThreads + threadIdx.x) * VEC; e < N; e += stride * VEC) {
+      if (e + VEC <= N) {
+        TA v[VEC];
+        vload<TA, VEC>(v, A + e);
+#pragma unroll
+        for (int i = 0; i < VEC; ++i) m = fmaxf(m, fabsf(alpha * to_f32(v[i])));
+      } else {
+        for (uint64_t i = e; i < N; ++i) m = fmaxf(m, fabsf(alpha * to_f32(A[i])));
+      }
+    }
+  } else {
+    for (uint64_t e = (uint64_t)blockIdx.x * kThreads + threadIdx.x; e < N; e += stride) {
+      uint32_t row = fdiv((uint32_t)e, q.dn0);
+      const uint32_t i0 = (uint32_t)e - row * (uint32_t)q.n0;
+      int64_t off = i0;
+#pragma unroll
+      for (int k = 1; k < kMaxDim; ++k) {
+        if (k < q.rank) {
+          const uint32_t qq = fdiv(row, q.dn[k]);
+          const uint32_t c = row - qq * q.dn[k].d;
+          row = qq;
+          off += (int64_t)c * q.sa[k];
+        }
+      }
+      m = fmaxf(m, fabsf(alpha * to_f32(A[off])));
+    }
+  }
+  block_amax_commit(m, amax_out);
+}
+
+}  // namespace ttc

@@ -1,0 +1,66 @@
+// Kernel registry: one entry per instantiated (type pair, vector widths, mode)
+// kernel, with a host launcher.  The instance files under gen/ are written by
+// ttc_amd/gen_instances.py and hold only lists of entries.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "ttc_kernels.hpp"
+
+namespace ttc {
+
+// mode of a transpose entry
+enum : int { kModeBeta = 0, kModeBz = 1, kModeBzNt = 2, kModeFp8 = 3 };
+
+using LaunchFn = hipError_t (*)(const TileParams&, const void*, void*, double, double, unsigned, size_t, hipStream_t);
+using LaunchFp8Fn = hipError_t (*)(const TileParams&, const void*, void*, float, const float*, const float*, float,
+                                   float, float*, float*, unsigned, size_t, hipStream_t);
+
+struct KernelEntry {
+  int ta, tb, va, vb, mode;
+  LaunchFn launch;
+  LaunchFp8Fn launch_fp8;
+  const void* func;
+};
+
+template <class OP, int VA, int VB, bool BETA, bool NT>
+hipError_t launch_transpose(const TileParams& p, const void* A, void* B, double alpha, double beta, unsigned grid,
+                            size_t lds, hipStream_t s) {
+  const typename OP::TA* a = static_cast<const typename OP::TA*>(A);
+  typename OP::TB* b = static_cast<typename OP::TB*>(B);
+  TileParams pp = p;
+  void* args[] = {&pp, &a, &b, &alpha, &beta};
+  return hipLaunchKernel(reinterpret_cast<const void*>(&transpose_kernel<OP, VA, VB, BETA, NT>), dim3(grid),
+                         dim3(kThreads), args, lds, s);
+}
+
+template <class OP, int VA, int VB>
+hipError_t launch_fp8(const TileParams& p, const void* A, void* B, float alpha, const float* scale_in,
+                      const float* amax_in, float fp8_max, float scale_host, float* amax_out, float* scale_inv_out,
+                      unsigned grid, size_t lds, hipStream_t s) {
+  const typename OP::TA* a = static_cast<const typename OP::TA*>(A);
+  typename OP::TB* b = static_cast<typename OP::TB*>(B);
+  TileParams pp = p;
+  void* args[] = {&pp, &a, &b, &alpha, &scale_in, &amax_in, &fp8_max, &scale_host, &amax_out, &scale_inv_out};
+  return hipLaunchKernel(reinterpret_cast<const void*>(&cast_fp8_kernel<OP, VA, VB>), dim3(grid), dim3(kThreads),
+                         args, lds, s);
+}
+
+template <class OP, int VA, int VB, int MODE>
+KernelEntry make_entry(int ta, int tb) {
+  KernelEntry e{ta, tb, VA, VB, MODE, nullptr, nullptr, nullptr};
+  if constexpr (MODE == kModeFp8) {
+    e.launch_fp8 = &launch_fp8<OP, VA, VB>;
+    e.func = reinterpret_cast<const void*>(&cast_fp8_kernel<OP, VA, VB>);
+  } else {
+    constexpr bool beta = MODE == kModeBeta;
+    constexpr bool nt = MODE == kModeBzNt;
+    e.launch = &launch_transpose<OP, VA, VB, beta, nt>;
+    e.func = reinterpret_cast<const void*>(&transpose_kernel<OP, VA, VB, beta, nt>);
+  }
+  return e;
+}
+
+// Registry access (ttc_runtime.hip).
+const KernelEntry* find_kernel(int ta, int tb, int va, int vb, int mode);
+
+}  // namespace ttc

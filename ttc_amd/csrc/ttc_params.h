@@ -1,0 +1,57 @@
+// Launch parameters shared by the host planner (ttc_plan.cpp) and the device
+// kernels (ttc_kernels.hpp).  Plain C++ so the planner builds without HIP.
+//
+// Geometry is that of the FUSED problem: rank r <= TTC_MAX_DIM, A-dimension k
+// has extent n[k], element stride sa in A and sb in B.  A tile is the box
+// t[0] x ... x t[r-1] (t[k] == 1 for dimensions the grid walks over, t[k] ==
+// n[k] for dimensions a tile covers fully, 1 < t[k] < n[k] for "split"
+// dimensions).  Tiles are numbered in A order (dimension 0 fastest) over the
+// tile grid g[k] = ceil(n[k] / t[k]).
+#pragma once
+#include <stdint.h>
+
+namespace ttc {
+
+constexpr int kMaxDim = 8;     // fused rank limit
+constexpr int kThreads = 256;  // work-group size (4 wave64)
+constexpr int kMaxElemsPerThread = 16;  // tile volume <= kThreads * 16
+
+// n / d for n < 2^31 via one multiply-high (round-up method): with
+// s = ceil(log2 d) and m = floor(2^32 (2^s - d) / d) + 1,
+// n / d = (mulhi(n, m) + n) >> s.
+struct FastDiv {
+  uint32_t d, m, s, pad_;
+};
+
+inline FastDiv make_fastdiv(uint32_t d) {
+  FastDiv f{};
+  f.d = d;
+  uint32_t s = 0;
+  while ((1ull << s) < d) ++s;
+  f.s = s;
+  f.m = (uint32_t)(((1ull << 32) * ((1ull << s) - d)) / d + 1);
+  return f;
+}
+
+struct TileParams {
+  int32_t rank;
+  int32_t V;            // elements in a full tile (prod t)
+  int32_t t[kMaxDim];   // tile extents
+  int32_t n[kMaxDim];   // tensor extents
+  int32_t perm[kMaxDim];       // B position q holds A dimension perm[q]
+  int32_t sa[kMaxDim];         // A element stride of dim k, used within a tile (int32 checked on host)
+  int32_t sb[kMaxDim];         // B element stride of dim k, used within a tile
+  int32_t ls[kMaxDim];         // LDS element stride of dim k
+  int64_t tsa[kMaxDim];        // t[k] * A stride: tile-to-tile step
+  int64_t tsb[kMaxDim];        // t[k] * B stride
+  FastDiv tdiv[kMaxDim];       // division by t[k]
+  FastDiv gdiv[kMaxDim];       // division by g[k] (tiles along dim k)
+  uint32_t ntiles;
+  int32_t split0, split1;      // dims with partial tiles (-1 = none)
+  int32_t tsplit0, tsplit1;    // t[split] (0 when none)
+  int32_t lds_stride_b;        // ls[perm[0]]: LDS distance of consecutive B-order elements
+  int32_t lds_elems;           // LDS elements used by one tile (with padding)
+  int32_t grid_order;          // 0 strided walk, 1 contiguous ranges
+};
+
+}  // namespace ttc

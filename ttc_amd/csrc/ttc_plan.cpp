@@ -1,0 +1,540 @@
+// Host-side planning for libttc_hip.so: validation, index fusion, tile choice
+// and kernel launch parameters.  No HIP runtime calls here.
+#include "ttc_plan.h"
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace ttc {
+
+int dtype_bytes(int t) {
+  switch (t) {
+    case TTC_S: return 4;
+    case TTC_D: return 8;
+    case TTC_C: return 8;
+    case TTC_Z: return 16;
+    case TTC_BF16:
+    case TTC_F16: return 2;
+    case TTC_E4M3:
+    case TTC_E5M2: return 1;
+    default: return 0;
+  }
+}
+
+static bool f32_class(int t) { return t == TTC_S || t == TTC_BF16 || t == TTC_F16; }
+
+bool pair_supported(int ta, int tb) {
+  if (f32_class(ta) && f32_class(tb)) return !(ta != TTC_S && tb != TTC_S && ta != tb);  // no bf16 <-> f16
+  if ((ta == TTC_S || ta == TTC_D) && (tb == TTC_S || tb == TTC_D)) return true;
+  if ((ta == TTC_C || ta == TTC_Z) && (tb == TTC_C || tb == TTC_Z)) return true;
+  if (f32_class(ta) && (tb == TTC_E4M3 || tb == TTC_E5M2)) return true;
+  return false;
+}
+
+int lds_elem_bytes(int ta, int /*tb*/) {
+  switch (ta) {
+    case TTC_D: return 8;
+    case TTC_C: return 8;
+    case TTC_Z: return 16;
+    default: return 4;  // s, bf16, f16: alpha*a is held in fp32
+  }
+}
+
+// ------------------------------------------------------------------ fusion
+// Restatement of ttc/ttc_util.py:214-276.  For every A index i, find the B
+// position j holding it and extend the match (A indices i, i+1, ... equal to
+// perm[j], perm[j+1], ...) while the previous dimension is unpadded in both A
+// (lda == size) and B (ldb == size); the matched run becomes one index whose
+// extent, lda and ldb are the products.  Afterwards the surviving index
+// labels are renumbered to 0..rank-1 keeping their order.
+ttc_status_t fuse_indices(int dim, const int* perm_in, const int* size_in, const int* lda_in, const int* ldb_in,
+                          int* out_dim, int* out_perm, int* out_size, int* out_lda, int* out_ldb) {
+  if (dim < 1 || dim > TTC_MAX_INPUT_DIM || !perm_in || !size_in) return TTC_ERROR_INVALID_VALUE;
+  std::vector<long long> size(dim), lda(dim), ldb(dim);
+  std::vector<int> perm(perm_in, perm_in + dim), idx(dim);
+  for (int k = 0; k < dim; ++k) {
+    if (perm[k] < 0 || perm[k] >= dim) return TTC_ERROR_INVALID_VALUE;
+    size[k] = size_in[k];
+    idx[k] = k;
+  }
+  for (int k = 0; k < dim; ++k) {
+    lda[k] = lda_in ? lda_in[k] : size[k];
+    ldb[k] = ldb_in ? ldb_in[k] : size[perm[k]];
+  }
+  for (int i = 0; i < dim; ++i) {
+    for (int j = 0; j < dim; ++j) {
+      int l = 0;
+      long long fused = 1;
+      while (i + l < (int)idx.size() && j + l < (int)perm.size() && idx[i + l] == perm[j + l] &&
+             (l == 0 || lda[i + l - 1] == size[i + l - 1]) && (l == 0 || ldb[j + l - 1] == size[i + l - 1])) {
+        fused *= size[i + l];
+        ++l;
+      }
+      if (l >= 1) size[i] = fused;
+      for (; l > 1; --l) {
+        idx.erase(idx.begin() + i + 1);
+        perm.erase(perm.begin() + j + 1);
+        size.erase(size.begin() + i + 1);
+        lda[i] *= lda[i + 1];
+        lda.erase(lda.begin() + i + 1);
+        ldb[j] *= ldb[j + 1];
+        ldb.erase(ldb.begin() + j + 1);
+      }
+    }
+  }
+  // Renumber perm labels to 0..rank-1 in increasing order (the labels left
+  // are distinct, so each becomes its rank among them).
+  const int r = (int)perm.size();
+  std::vector<int> sorted(perm);
+  std::sort(sorted.begin(), sorted.end());
+  for (int q = 0; q < r; ++q)
+    perm[q] = (int)(std::lower_bound(sorted.begin(), sorted.end(), perm[q]) - sorted.begin());
+  for (int k = 0; k < r; ++k)
+    if (size[k] > INT_MAX || lda[k] > INT_MAX || ldb[k] > INT_MAX) return TTC_ERROR_UNSUPPORTED;
+  *out_dim = r;
+  for (int k = 0; k < r; ++k) {
+    out_perm[k] = perm[k];
+    out_size[k] = (int)size[k];
+    if (out_lda) out_lda[k] = (int)lda[k];
+    if (out_ldb) out_ldb[k] = (int)ldb[k];
+  }
+  return TTC_SUCCESS;
+}
+
+ttc_status_t make_geometry(int dim, const int* perm, const int* size, const int* lda, const int* ldb, Geometry& g) {
+  if (dim < 1 || dim > TTC_MAX_INPUT_DIM || !perm || !size) return TTC_ERROR_INVALID_VALUE;
+  bool seen[TTC_MAX_INPUT_DIM] = {false};
+  for (int k = 0; k < dim; ++k) {  // ttc/ttc.py:753-757
+    if (perm[k] < 0 || perm[k] >= dim || seen[perm[k]]) return TTC_ERROR_INVALID_VALUE;
+    seen[perm[k]] = true;
+    if (size[k] < 0) return TTC_ERROR_INVALID_VALUE;
+  }
+  for (int k = 0; k < dim; ++k) {  // ttc/ttc.py:739-746
+    if (lda && lda[k] < size[k]) return TTC_ERROR_INVALID_VALUE;
+    if (ldb && ldb[k] < size[perm[k]]) return TTC_ERROR_INVALID_VALUE;
+  }
+  int r = 0;
+  int fperm[TTC_MAX_INPUT_DIM], fsize[TTC_MAX_INPUT_DIM], flda[TTC_MAX_INPUT_DIM], fldb[TTC_MAX_INPUT_DIM];
+  ttc_status_t st = fuse_indices(dim, perm, size, lda, ldb, &r, fperm, fsize, flda, fldb);
+  if (st != TTC_SUCCESS) return st;
+  if (r > kMaxDim) return TTC_ERROR_UNSUPPORTED;
+  g = Geometry();
+  g.rank = r;
+  g.elements = 1;
+  for (int k = 0; k < r; ++k) {
+    g.perm[k] = fperm[k];
+    g.size[k] = fsize[k];
+    g.lda[k] = flda[k];
+    g.ldb[k] = fldb[k];
+    g.elements *= fsize[k];
+  }
+  int64_t sbpos[kMaxDim];
+  g.sa[0] = 1;
+  sbpos[0] = 1;
+  for (int k = 1; k < r; ++k) {
+    g.sa[k] = g.sa[k - 1] * flda[k - 1];
+    sbpos[k] = sbpos[k - 1] * fldb[k - 1];
+  }
+  for (int q = 0; q < r; ++q) g.sb[fperm[q]] = sbpos[q];
+  return TTC_SUCCESS;
+}
+
+// ------------------------------------------------------------ vector widths
+static bool vec_ok_a(const Geometry& g, int w) {
+  if (g.size[0] % w) return false;
+  for (int k = 1; k < g.rank; ++k)
+    if (g.sa[k] % w) return false;
+  return true;
+}
+static bool vec_ok_b(const Geometry& g, int w) {
+  const int p0 = g.perm[0];
+  if (g.size[p0] % w) return false;
+  for (int k = 0; k < g.rank; ++k)
+    if (k != p0 && g.sb[k] % w) return false;
+  return true;
+}
+int widest_vec_a(const Geometry& g, int ta) {
+  int best = 1;
+  for (int w : vec_widths(dtype_bytes(ta)))
+    if (vec_ok_a(g, w)) best = std::max(best, w);
+  return best;
+}
+int widest_vec_b(const Geometry& g, int tb) {
+  int best = 1;
+  for (int w : vec_widths(dtype_bytes(tb)))
+    if (vec_ok_b(g, w)) best = std::max(best, w);
+  return best;
+}
+
+int max_tile_volume(int va, int vb, int lds_elem) {
+  const int kLdsBudget = 32768;  // bytes of LDS per work-group (5 work-groups per 160 KiB CU)
+  int v = kThreads * std::min(slots_for(va) * va, slots_for(vb) * vb);
+  return std::min(v, kLdsBudget / lds_elem);
+}
+
+// -------------------------------------------------------------- tile choice
+static int round_up(int x, int m) { return (x + m - 1) / m * m; }
+
+// Split extent n into pieces of about `want`, balanced so the last piece is
+// not a sliver, then rounded up to `align` (which divides n).
+static int split_extent(int n, int want, int align) {
+  want = std::max(want, 1);
+  if (want >= n) return n;
+  const int pieces = (n + want - 1) / want;
+  int t = (n + pieces - 1) / pieces;
+  t = round_up(t, align);
+  return std::min(t, n);
+}
+
+static int align_of(const Geometry& g, int k, int va, int vb) {
+  int a = 1;
+  if (k == 0) a = std::max(a, va);
+  if (k == g.perm[0]) a = std::max(a, vb);
+  return a;
+}
+
+// Tile for given run targets (elements) on A's and B's side.
+static void tile_for_runs(const Geometry& g, int va, int vb, int vmax, int run_a, int run_b, int* tile) {
+  const int r = g.rank;
+  for (int k = 0; k < kMaxDim; ++k) tile[k] = 1;
+  // A side: cover A's leading dimensions while they stay one contiguous run.
+  long long prod = 1;
+  for (int k = 0; k < r; ++k) {
+    if (k > 0 && g.sa[k] != g.sa[k - 1] * g.size[k - 1]) break;
+    const int n = g.size[k];
+    if (prod * n <= run_a) {
+      tile[k] = n;
+      prod *= n;
+      continue;
+    }
+    tile[k] = std::max(tile[k], split_extent(n, (int)((run_a + prod - 1) / prod), align_of(g, k, va, vb)));
+    break;
+  }
+  // B side: the same along B's order.
+  prod = 1;
+  for (int q = 0; q < r; ++q) {
+    const int k = g.perm[q];
+    if (q > 0 && g.sb[k] != g.sb[g.perm[q - 1]] * g.size[g.perm[q - 1]]) break;
+    const int n = g.size[k];
+    if (tile[k] == n || prod * n <= run_b) {
+      tile[k] = n;
+      prod *= n;
+      continue;
+    }
+    tile[k] = std::max(tile[k], split_extent(n, (int)((run_b + prod - 1) / prod), align_of(g, k, va, vb)));
+    break;
+  }
+  for (int k = 0; k < r; ++k)
+    tile[k] = std::max(1, std::min(round_up(tile[k], align_of(g, k, va, vb)), std::max(g.size[k], 1)));
+  // Shrink to the volume limit: halve the largest extent that can shrink.
+  for (;;) {
+    long long v = 1;
+    for (int k = 0; k < r; ++k) v *= tile[k];
+    if (v <= vmax) break;
+    int best = -1;
+    for (int k = 0; k < r; ++k) {
+      const int a = align_of(g, k, va, vb);
+      if (tile[k] > a && (best < 0 || tile[k] >= tile[best])) best = k;
+    }
+    if (best < 0) break;
+    const int a = align_of(g, best, va, vb);
+    tile[best] = std::max(a, round_up((tile[best] + 1) / 2, a));
+  }
+  // At most two dimensions may have a partial last tile; make any further
+  // split extent a divisor of its dimension.
+  int partial = 0;
+  for (int k = 0; k < r; ++k) {
+    if (g.size[k] % tile[k] == 0) continue;
+    if (++partial <= 2) continue;
+    const int a = align_of(g, k, va, vb);
+    int t = tile[k];
+    while (t > a && (g.size[k] % t != 0 || t % a != 0)) --t;
+    tile[k] = std::max(t, a);
+  }
+}
+
+void tile_runs(const Geometry& g, const int* tile, long long& ra, long long& rb) {
+  ra = 1;
+  for (int k = 0; k < g.rank; ++k) {
+    ra *= tile[k];
+    if (tile[k] != g.size[k] || (k + 1 < g.rank && g.sa[k + 1] != g.sa[k] * g.size[k])) break;
+  }
+  rb = 1;
+  for (int q = 0; q < g.rank; ++q) {
+    const int k = g.perm[q];
+    rb *= tile[k];
+    if (tile[k] != g.size[k] || (q + 1 < g.rank && g.sb[g.perm[q + 1]] != g.sb[k] * g.size[k])) break;
+  }
+}
+
+// Model of a tile's efficiency: the fraction of launched tile slots that hold
+// data (partial edge tiles waste the rest), times b/(b+64) per side for a
+// contiguous run of b bytes (a misaligned run drags in about half a 128-byte
+// line at each end), times V/(V+256) for the per-tile fixed cost.
+double tile_score(const Geometry& g, const int* tile, int szA, int szB) {
+  long long V = 1, ntiles = 1;
+  for (int k = 0; k < g.rank; ++k) {
+    V *= tile[k];
+    ntiles *= (g.size[k] + tile[k] - 1) / std::max(tile[k], 1);
+  }
+  if (V <= 0 || ntiles <= 0) return 0.0;
+  long long ra, rb;
+  tile_runs(g, tile, ra, rb);
+  const double eff = (double)g.elements / ((double)ntiles * (double)V);
+  const double ba = (double)ra * szA, bb = (double)rb * szB;
+  return eff * (ba / (ba + 64.0)) * (bb / (bb + 64.0)) * ((double)V / ((double)V + 256.0));
+}
+
+void choose_tile(const Geometry& g, int ta, int tb, int va, int vb, int run_a, int run_b, int* tile) {
+  const int szA = dtype_bytes(ta), szB = dtype_bytes(tb);
+  const int vmax = max_tile_volume(va, vb, lds_elem_bytes(ta, tb));
+  if (run_a > 0 && run_b > 0) {
+    tile_for_runs(g, va, vb, vmax, run_a, run_b, tile);
+    return;
+  }
+  // Search the run targets (powers of two) and keep the best-scoring tile.
+  double best = -1.0;
+  int cand[kMaxDim];
+  for (int ra = 1; ra <= vmax; ra *= 2) {
+    for (int rb = 1; rb <= vmax; rb *= 2) {
+      if ((long long)ra * rb > 4LL * vmax) continue;
+      tile_for_runs(g, va, vb, vmax, run_a > 0 ? run_a : ra, run_b > 0 ? run_b : rb, cand);
+      const double sc = tile_score(g, cand, szA, szB);
+      if (sc > best * (1.0 + 1e-9)) {
+        best = sc;
+        for (int k = 0; k < kMaxDim; ++k) tile[k] = cand[k];
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------ launch params
+ttc_status_t build_params(const Geometry& g, const int* tile, int va, int vb, int lds_elem, int grid_order,
+                          TileParams& p, int& lds_bytes, int& lds_pad) {
+  std::memset(&p, 0, sizeof(p));
+  const int r = g.rank;
+  p.rank = r;
+  long long V = 1, ntiles = 1, spanA = 0, spanB = 0;
+  int nsplit = 0;
+  p.split0 = p.split1 = -1;
+  for (int k = 0; k < r; ++k) {
+    const int n = g.size[k], t = tile[k];
+    if (n == 0) {
+      ntiles = 0;
+    }
+    if (t < 1 || (n > 0 && t > n) || t > 65535) return TTC_ERROR_INVALID_VALUE;
+    V *= t;
+    spanA += (long long)(t - 1) * g.sa[k];
+    spanB += (long long)(t - 1) * g.sb[k];
+    p.t[k] = t;
+    p.n[k] = n;
+    p.perm[k] = g.perm[k];
+    p.sa[k] = t > 1 ? (int32_t)g.sa[k] : 0;
+    p.sb[k] = t > 1 ? (int32_t)g.sb[k] : 0;
+    p.tsa[k] = (int64_t)t * g.sa[k];
+    p.tsb[k] = (int64_t)t * g.sb[k];
+    p.tdiv[k] = make_fastdiv((uint32_t)t);
+    const long long gk = n > 0 ? (n + t - 1) / t : 1;
+    ntiles *= gk;
+    if (ntiles > INT_MAX) return TTC_ERROR_UNSUPPORTED;
+    p.gdiv[k] = make_fastdiv((uint32_t)gk);
+    if (n > 0 && n % t != 0) {
+      if (nsplit == 0) { p.split0 = k; p.tsplit0 = t; }
+      else if (nsplit == 1) { p.split1 = k; p.tsplit1 = t; }
+      else return TTC_ERROR_UNSUPPORTED;
+      ++nsplit;
+    }
+  }
+  if (spanA >= INT_MAX || spanB >= INT_MAX) return TTC_ERROR_UNSUPPORTED;
+  if (V > max_tile_volume(va, vb, lds_elem)) return TTC_ERROR_UNSUPPORTED;
+  const int p0 = g.perm[0];
+  if (tile[0] % va || tile[p0] % vb) return TTC_ERROR_INVALID_VALUE;
+  if (g.size[0] % va || g.size[p0] % vb) return TTC_ERROR_INVALID_VALUE;
+  p.V = (int32_t)V;
+  p.ntiles = (uint32_t)ntiles;
+  // LDS layout: A order; the stride of B's stride-1 dimension is made odd so
+  // that lane-consecutive reads of the write phase hit distinct banks.
+  lds_pad = 0;
+  p.ls[0] = 1;
+  for (int k = 1; k < r; ++k) {
+    p.ls[k] = p.ls[k - 1] * tile[k - 1];
+    if (k == p0 && (p.ls[k] % 2) == 0) {
+      lds_pad = 1;
+      p.ls[k] += 1;
+    }
+  }
+  long long elems = 1;
+  for (int k = 0; k < r; ++k) elems += (long long)(tile[k] - 1) * p.ls[k];
+  p.lds_elems = (int32_t)elems;
+  p.lds_stride_b = p.ls[p0];
+  lds_bytes = (int)((elems * lds_elem + 15) / 16 * 16);
+  if (lds_bytes > 65536) return TTC_ERROR_UNSUPPORTED;
+  p.grid_order = grid_order == 2 ? 1 : 0;
+  return TTC_SUCCESS;
+}
+
+}  // namespace ttc
+
+// ================================================================== C ABI
+using namespace ttc;
+
+extern "C" const char* ttc_version(void) { return "ttc-mi355x 0.1.0 (gfx950)"; }
+
+extern "C" const char* ttc_status_string(ttc_status_t s) {
+  switch (s) {
+    case TTC_SUCCESS: return "success";
+    case TTC_ERROR_INVALID_VALUE: return "invalid value";
+    case TTC_ERROR_UNSUPPORTED: return "unsupported type pair or shape";
+    case TTC_ERROR_HIP: return "HIP runtime error";
+    case TTC_ERROR_ALLOC: return "allocation failed";
+    case TTC_ERROR_NO_DEVICE: return "no usable gfx950 device";
+  }
+  return "unknown status";
+}
+
+extern "C" int ttc_dtype_size(ttc_dtype_t t) { return dtype_bytes((int)t); }
+
+extern "C" ttc_status_t ttc_fuse_indices(int dim, const int* perm, const int* size, const int* lda, const int* ldb,
+                                         int* out_dim, int* out_perm, int* out_size, int* out_lda, int* out_ldb) {
+  if (!out_dim || !out_perm || !out_size) return TTC_ERROR_INVALID_VALUE;
+  return fuse_indices(dim, perm, size, lda, ldb, out_dim, out_perm, out_size, out_lda, out_ldb);
+}
+
+static ttc_status_t plan_fill(ttc_plan_s* pl, int va, int vb, const int* tile_in, int run_a, int run_b) {
+  const Geometry& g = pl->g;
+  int tile[kMaxDim];
+  bool explicit_tile = false;
+  if (tile_in)
+    for (int k = 0; k < g.rank; ++k) explicit_tile |= tile_in[k] != 0;
+  if (explicit_tile) {
+    for (int k = 0; k < kMaxDim; ++k) tile[k] = k < g.rank ? tile_in[k] : 1;
+  } else {
+    choose_tile(g, pl->type_a, pl->type_b, va, vb, run_a, run_b, tile);
+  }
+  pl->va = va;
+  pl->vb = vb;
+  for (int k = 0; k < kMaxDim; ++k) pl->tile[k] = k < g.rank ? tile[k] : 1;
+  ttc_status_t st = build_params(g, tile, va, vb, lds_elem_bytes(pl->type_a, pl->type_b), pl->grid_order, pl->p,
+                                 pl->lds_bytes, pl->lds_pad);
+  if (st != TTC_SUCCESS) return st;
+  long long ra, rb;
+  tile_runs(g, tile, ra, rb);
+  pl->run_a = (int)std::min<long long>(ra, INT_MAX);
+  pl->run_b = (int)std::min<long long>(rb, INT_MAX);
+  return TTC_SUCCESS;
+}
+
+extern "C" ttc_status_t ttc_plan_create(ttc_plan_t* plan, ttc_dtype_t type_a, ttc_dtype_t type_b, int dim,
+                                        const int* perm, const int* size, const int* lda, const int* ldb,
+                                        int beta_is_zero, const ttc_config_t* cfg) {
+  if (!plan) return TTC_ERROR_INVALID_VALUE;
+  *plan = nullptr;
+  if (!pair_supported(type_a, type_b)) return TTC_ERROR_UNSUPPORTED;
+  const bool fp8 = type_b == TTC_E4M3 || type_b == TTC_E5M2;
+  if (fp8 && !beta_is_zero) return TTC_ERROR_UNSUPPORTED;
+  ttc_config_t c;
+  std::memset(&c, 0, sizeof(c));
+  if (cfg) c = *cfg;
+  if (c.nt_store < 0 || c.nt_store > 2 || c.grid_order < 0 || c.grid_order > 2 || c.wg_per_cu < 0 || c.vec_a < 0 ||
+      c.vec_b < 0)
+    return TTC_ERROR_INVALID_VALUE;
+  ttc_plan_s* pl = new (std::nothrow) ttc_plan_s();
+  if (!pl) return TTC_ERROR_ALLOC;
+  pl->type_a = type_a;
+  pl->type_b = type_b;
+  pl->beta_is_zero = beta_is_zero ? 1 : 0;
+  pl->fallback = nullptr;
+  ttc_status_t st = make_geometry(dim, perm, size, lda, ldb, pl->g);
+  if (st != TTC_SUCCESS) {
+    delete pl;
+    return st;
+  }
+  pl->nt_store = c.nt_store == 0 ? (pl->beta_is_zero ? 1 : 0) : (c.nt_store == 1 ? 1 : 0);
+  if (!pl->beta_is_zero) pl->nt_store = 0;  // B is read back: keep it cacheable
+  pl->grid_order = c.grid_order;
+  pl->wg_per_cu = c.wg_per_cu;
+  auto in_list = [](int w, int bytes) {
+    for (int x : vec_widths(bytes))
+      if (x == w) return true;
+    return false;
+  };
+  int va = c.vec_a ? c.vec_a : widest_vec_a(pl->g, type_a);
+  int vb = c.vec_b ? c.vec_b : widest_vec_b(pl->g, type_b);
+  if (!in_list(va, dtype_bytes(type_a)) || !in_list(vb, dtype_bytes(type_b)) || !vec_ok_a(pl->g, va) ||
+      !vec_ok_b(pl->g, vb)) {
+    delete pl;
+    return TTC_ERROR_INVALID_VALUE;
+  }
+  st = plan_fill(pl, va, vb, c.tile, c.run_a, c.run_b);
+  if (st != TTC_SUCCESS) {
+    delete pl;
+    return st;
+  }
+  if (va > 1 || vb > 1) {
+    // Scalar-vector variant for pointers that are not aligned to the vector.
+    ttc_plan_s* fb = new (std::nothrow) ttc_plan_s(*pl);
+    if (!fb) {
+      delete pl;
+      return TTC_ERROR_ALLOC;
+    }
+    fb->fallback = nullptr;
+    const int* t = pl->tile;
+    long long v = 1;
+    for (int k = 0; k < pl->g.rank; ++k) v *= t[k];
+    const bool reuse = v <= max_tile_volume(1, 1, lds_elem_bytes(type_a, type_b));
+    st = plan_fill(fb, 1, 1, reuse ? t : nullptr, c.run_a, c.run_b);
+    if (st != TTC_SUCCESS) {
+      delete fb;
+      delete pl;
+      return st;
+    }
+    pl->fallback = fb;
+  }
+  *plan = pl;
+  return TTC_SUCCESS;
+}
+
+extern "C" ttc_status_t ttc_plan_get_info(ttc_plan_t pl, ttc_plan_info_t* info) {
+  if (!pl || !info) return TTC_ERROR_INVALID_VALUE;
+  std::memset(info, 0, sizeof(*info));
+  const Geometry& g = pl->g;
+  info->rank = g.rank;
+  for (int k = 0; k < g.rank; ++k) {
+    info->perm[k] = g.perm[k];
+    info->size[k] = g.size[k];
+    info->stride_a[k] = g.sa[k];
+    info->stride_b[k] = g.sb[k];
+    info->tile[k] = pl->tile[k];
+  }
+  info->vec_a = pl->va;
+  info->vec_b = pl->vb;
+  info->threads = kThreads;
+  info->slots_a = (pl->p.V / pl->va + kThreads - 1) / kThreads;
+  info->slots_b = (pl->p.V / pl->vb + kThreads - 1) / kThreads;
+  info->tiles = pl->p.ntiles;
+  info->tile_volume = pl->p.V;
+  info->lds_bytes = pl->lds_bytes;
+  info->lds_pad = pl->lds_pad;
+  info->nt_store = pl->nt_store;
+  info->grid_order = pl->p.grid_order;
+  info->wg_per_cu = pl->wg_per_cu;
+  info->elements = g.elements;
+  const int sa = dtype_bytes(pl->type_a), sb = dtype_bytes(pl->type_b);
+  info->bytes_algorithmic = g.elements * (int64_t)sa + g.elements * (int64_t)sb * (pl->beta_is_zero ? 1 : 2);
+  info->type_a = pl->type_a;
+  info->type_b = pl->type_b;
+  info->beta_is_zero = pl->beta_is_zero;
+  info->run_a = pl->run_a;
+  info->run_b = pl->run_b;
+  return TTC_SUCCESS;
+}
+
+extern "C" ttc_status_t ttc_plan_destroy(ttc_plan_t pl) {
+  if (!pl) return TTC_ERROR_INVALID_VALUE;
+  delete pl->fallback;
+  delete pl;
+  return TTC_SUCCESS;
+}

@@ -1,0 +1,87 @@
+// Host-side planning: validation, index fusion, tile choice.  No HIP calls,
+// so plans can be made (and inspected) without a device.
+#pragma once
+#include <stdint.h>
+
+#include <vector>
+
+#include "../../include/ttc_hip.h"
+#include "ttc_params.h"
+
+namespace ttc {
+
+// Vector widths (elements per lane) instantiated for an element of `bytes`
+// bytes: 1, 2 and the 16-byte width.  The kernel generator uses the same rule.
+inline std::vector<int> vec_widths(int bytes) {
+  std::vector<int> w{1};
+  if (bytes <= 8) w.push_back(2);
+  const int wide = 16 / bytes;
+  if (wide > 2) w.push_back(wide);
+  return w;
+}
+// Slots (vectors per thread per tile side) for a vector width.
+constexpr int slots_for(int vec) { return vec >= 4 ? 4 : kMaxElemsPerThread / vec; }
+
+int dtype_bytes(int t);
+// Bytes of the LDS staging type (alpha * a in A's arithmetic) for a type pair.
+int lds_elem_bytes(int ta, int tb);
+bool pair_supported(int ta, int tb);
+
+struct Geometry {
+  int rank = 0;
+  int perm[kMaxDim] = {0};
+  int size[kMaxDim] = {0};
+  int64_t sa[kMaxDim] = {0};  // A stride of A-dim k
+  int64_t sb[kMaxDim] = {0};  // B stride of A-dim k
+  int lda[kMaxDim] = {0};     // fused outer extents
+  int ldb[kMaxDim] = {0};
+  int64_t elements = 0;
+};
+
+// Restatement of ttc/ttc_util.py:214-276 on fixed arrays.
+ttc_status_t fuse_indices(int dim, const int* perm, const int* size, const int* lda, const int* ldb, int* out_dim,
+                          int* out_perm, int* out_size, int* out_lda, int* out_ldb);
+
+// Validation (ttc/ttc.py:733-768) + fusion + strides.
+ttc_status_t make_geometry(int dim, const int* perm, const int* size, const int* lda, const int* ldb, Geometry& g);
+
+struct PlanChoice {
+  int tile[kMaxDim];
+  int va, vb;
+  int lds_pad;
+  int run_a, run_b;
+};
+
+// Fills TileParams for a geometry and a tile; returns TTC_ERROR_UNSUPPORTED
+// when the tile violates a kernel limit.
+ttc_status_t build_params(const Geometry& g, const int* tile, int va, int vb, int lds_elem_bytes, int grid_order,
+                          TileParams& p, int& lds_bytes, int& lds_pad);
+
+// Contiguous element runs of a full tile in A and in B.
+void tile_runs(const Geometry& g, const int* tile, long long& ra, long long& rb);
+double tile_score(const Geometry& g, const int* tile, int szA, int szB);
+// Heuristic tile for a geometry.
+void choose_tile(const Geometry& g, int ta, int tb, int va, int vb, int run_a, int run_b, int* tile);
+
+// Widest legal vector on each side for the geometry (alignment of the
+// pointers is checked at execution).
+int widest_vec_a(const Geometry& g, int ta);
+int widest_vec_b(const Geometry& g, int tb);
+// Largest tile volume the kernel takes for these vector widths / LDS type.
+int max_tile_volume(int va, int vb, int lds_elem_bytes);
+
+}  // namespace ttc
+
+struct ttc_plan_s {
+  int type_a, type_b, beta_is_zero;
+  ttc::Geometry g;
+  int va, vb;
+  int nt_store;
+  int wg_per_cu;
+  int grid_order;
+  int tile[ttc::kMaxDim];
+  ttc::TileParams p;
+  int lds_bytes, lds_pad;
+  int run_a, run_b;
+  ttc_plan_s* fallback;  // scalar-vector plan for misaligned pointers (may be null)
+};
