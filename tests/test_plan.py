@@ -1,0 +1,166 @@
+"""CPU tests of the host side of libttc_hip.so: the library loads and
+exports every symbol of include/ttc_hip.h, index fusion agrees with the
+Python restatement of ttc/ttc_util.py:214-276 and leaves the result of the
+operation unchanged, and plans satisfy the kernels' constraints.  No compute
+is launched (there is no GPU here)."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+from common import CODE, D, S, make_case, oracle_run, same_bits
+from ttc_amd import runtime as rt
+from ttc_amd import ttc_util
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    txt = open(os.path.join(ROOT, "include", "ttc_hip.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?[a-z_]+[\s\*]+\**(ttc_[a-z0-9_]+)\s*\(", txt, flags=re.M)))
+
+
+def test_library_exports_every_header_symbol():
+    import ctypes
+    names = header_functions()
+    assert len(names) >= 16
+    L = ctypes.CDLL(rt.LIB_PATH)
+    for n in names:
+        assert hasattr(L, n), n
+    assert set(names) == set(rt.EXPORTS)
+
+
+def test_version_and_sizes():
+    assert "gfx950" in rt.version()
+    L = rt.lib()
+    assert [L.ttc_dtype_size(t) for t in range(8)] == [4, 8, 8, 16, 2, 2, 1, 1]
+
+
+def _py_fuse(perm, size, lda, ldb):
+    s, p = list(size), list(perm)
+    la, lb = list(lda or []), list(ldb or [])
+    ttc_util.fuseIndices(s, p, [], la, lb)
+    return p, s, la, lb
+
+
+def random_problem(rng, max_rank=7):
+    r = int(rng.integers(1, max_rank + 1))
+    size = [int(x) for x in rng.integers(1, 6, r)]
+    perm = [int(x) for x in rng.permutation(r)]
+    if rng.random() < 0.5:  # make fusable runs likely
+        k = int(rng.integers(0, r))
+        perm = sorted(perm[:k]) + perm[k:]
+    lda = [s + int(rng.integers(0, 2)) for s in size] if rng.random() < 0.3 else None
+    ldb = [size[perm[q]] + int(rng.integers(0, 2)) for q in range(r)] if rng.random() < 0.3 else None
+    return perm, size, lda, ldb
+
+
+def test_fusion_matches_python_restatement():
+    rng = np.random.default_rng(1)
+    for _ in range(400):
+        perm, size, lda, ldb = random_problem(rng)
+        cp, cs, cla, clb = rt.fuse_indices(perm, size, lda, ldb)
+        pp, ps, pla, plb = _py_fuse(perm, size, lda, ldb)
+        assert cp == pp and cs == ps, (perm, size, lda, ldb)
+        if lda:
+            assert cla == pla
+        if ldb:
+            assert clb == plb
+
+
+def test_fusion_known_cases():
+    # contiguous in both -> one index (the identity is a copy)
+    assert rt.fuse_indices([0, 1, 2], [4, 5, 6])[:2] == ([0], [120])
+    # (0,1) move together
+    assert rt.fuse_indices([2, 0, 1], [4, 5, 6])[:2] == ([1, 0], [20, 6])
+    # padding in A breaks the run
+    assert rt.fuse_indices([2, 0, 1], [4, 5, 6], [5, 5, 6], None)[:2] == ([2, 0, 1], [4, 5, 6])
+    # padding on the LAST index of a run does not
+    assert rt.fuse_indices([2, 0, 1], [4, 5, 6], [4, 7, 6], None)[:2] == ([1, 0], [20, 6])
+    # configs of the benchmark are left alone
+    assert rt.fuse_indices([1, 0, 2], [1000, 768, 16])[:2] == ([1, 0, 2], [1000, 768, 16])
+    assert rt.fuse_indices([3, 1, 0, 2], [256, 192, 128, 96])[:2] == ([3, 1, 0, 2], [256, 192, 128, 96])
+
+
+def test_fusion_is_idempotent_and_preserves_the_result(oracle_lib):
+    rng = np.random.default_rng(2)
+    for i in range(120):
+        perm, size, lda, ldb = random_problem(rng, 6)
+        fp, fs, fla, flb = rt.fuse_indices(perm, size, lda, ldb)
+        assert rt.fuse_indices(fp, fs, fla, flb)[:2] == (fp, fs)
+        A, B = make_case(S, S, perm, size, lda, ldb, seed=i, dist="normal")
+        want = oracle_run(S, S, A, B, 1.5, -0.5, perm, size, lda, ldb)
+        got = oracle_run(S, S, A, B, 1.5, -0.5, fp, fs, fla if lda else None, flb if ldb else None)
+        assert same_bits(got, want), (perm, size, lda, ldb, fp, fs)
+
+
+def _check_plan(i, perm, size):
+    r = i["rank"]
+    assert r == len(i["perm"]) and sorted(i["perm"]) == list(range(r))
+    V = 1
+    tiles = 1
+    for k in range(r):
+        assert 1 <= i["tile"][k] <= max(1, i["size"][k])
+        V *= i["tile"][k]
+        tiles *= -(-i["size"][k] // i["tile"][k]) if i["size"][k] else 0
+    assert V == i["tile_volume"]
+    assert tiles == i["tiles"] or i["elements"] == 0
+    assert i["tile"][0] % i["vec_a"] == 0 and i["tile"][i["perm"][0]] % i["vec_b"] == 0
+    assert i["size"][0] % i["vec_a"] == 0 and i["size"][i["perm"][0]] % i["vec_b"] == 0
+    assert i["slots_a"] <= (16 if i["vec_a"] < 4 else 4) and i["slots_b"] <= (16 if i["vec_b"] < 4 else 4)
+    assert 0 < i["lds_bytes"] <= 65536
+    partial = sum(1 for k in range(r) if i["size"][k] % i["tile"][k])
+    assert partial <= 2
+    assert i["elements"] == int(np.prod(size))
+
+
+@pytest.mark.parametrize("code", ["s", "d", "c", "z", "sd", "ds", "cz", "zc", "b", "h", "sb", "sh", "bs", "hs"])
+def test_plans_satisfy_kernel_constraints(code):
+    tA, tB = CODE[code]
+    rng = np.random.default_rng(3)
+    for trial in range(150):
+        r = int(rng.integers(1, 7))
+        size = [int(x) for x in np.exp(rng.uniform(0, np.log(3000), r)).astype(int) + 1]
+        while np.prod(size, dtype=np.float64) > 2e8:
+            size[int(np.argmax(size))] //= 2
+        perm = [int(x) for x in rng.permutation(r)]
+        bz = bool(trial % 2)
+        plan = rt.Plan(tA, tB, perm, size, beta_is_zero=bz)
+        i = plan.info()
+        _check_plan(i, perm, size)
+        sa, sb = rt.DTYPE_BYTES[tA], rt.DTYPE_BYTES[tB]
+        n = i["elements"]
+        assert i["bytes_algorithmic"] == n * sa + n * sb * (1 if bz else 2)
+
+
+def test_explicit_config_is_honoured():
+    plan = rt.Plan(S, S, [1, 0, 2], [1000, 768, 16], config=dict(tile=[32, 64, 1], vec_a=2, vec_b=1))
+    i = plan.info()
+    assert i["tile"] == [32, 64, 1] and i["vec_a"] == 2 and i["vec_b"] == 1
+
+
+def test_invalid_arguments_are_rejected():
+    with pytest.raises(rt.TTCError) as e:
+        rt.Plan(S, S, [0, 0], [3, 4])
+    assert e.value.status == rt.ERROR_INVALID_VALUE
+    with pytest.raises(rt.TTCError):
+        rt.Plan(S, S, [1, 0], [3, 4], lda=[2, 4])       # lda < size (ttc/ttc.py:743-746)
+    with pytest.raises(rt.TTCError):
+        rt.Plan(S, S, [1, 0], [3, 4], ldb=[3, 4])       # ldb[0] < size[perm[0]] (ttc/ttc.py:739-742)
+    with pytest.raises(rt.TTCError) as e:
+        rt.Plan(D, rt.BF16, [1, 0], [3, 4])
+    assert e.value.status == rt.ERROR_UNSUPPORTED
+    with pytest.raises(rt.TTCError):
+        rt.Plan(S, rt.E4M3, [1, 0], [3, 4], beta_is_zero=False)  # fp8 output only with beta == 0
+    with pytest.raises(rt.TTCError):
+        rt.Plan(S, S, [1, 0], [6, 4], config=dict(vec_a=4))   # 6 % 4 != 0
+    with pytest.raises(rt.TTCError):
+        rt.Plan(S, S, [1, 0], [100, 100], config=dict(tile=[100, 100]))  # volume above the kernel limit
+
+
+def test_too_many_partial_dims_is_unsupported():
+    with pytest.raises(rt.TTCError) as e:
+        rt.Plan(S, S, [2, 1, 0], [10, 10, 10], config=dict(tile=[3, 3, 3], vec_a=1, vec_b=1))
+    assert e.value.status == rt.ERROR_UNSUPPORTED

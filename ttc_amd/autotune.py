@@ -1,0 +1,132 @@
+"""On-device search over kernel configurations.
+
+TTC's method — enumerate blockings x loop orders, prune to about
+sqrt(maxImplementations) of each by cost models, compile and time every
+survivor, keep the fastest (ttc/GPUtransposeGenerator.py:176-230, 328-349;
+ttc/ttc.py:871-1022) — is kept, with the search space of this kernel: tile
+shape (through run-length targets on A's and B's side), vector width per
+side, persistent work-groups per CU, tile-walk order and non-temporal
+stores.  Nothing is compiled per candidate: every candidate is a launch of a
+prebuilt kernel with different arguments, timed with HIP events.
+"""
+import math
+
+from . import runtime as rt
+
+
+def tile_score(info, size_a, size_b):
+    """Same model as the planner (csrc/ttc_plan.cpp tile_score): fraction of
+    useful tile slots x b/(b+64) per side for contiguous runs of b bytes x
+    V/(V+256)."""
+    n = info["elements"]
+    if n == 0:
+        return 0.0
+    eff = n / float(info["tiles"] * info["tile_volume"])
+    ba, bb = info["run_a"] * size_a, info["run_b"] * size_b
+    v = info["tile_volume"]
+    return eff * (ba / (ba + 64.0)) * (bb / (bb + 64.0)) * (v / (v + 256.0))
+
+
+def candidates(type_a, type_b, perm, size, lda=None, ldb=None, beta_is_zero=False, max_impl=200):
+    """List of config dicts, best-first by the model, at most max_impl."""
+    base = rt.Plan(type_a, type_b, perm, size, lda, ldb, beta_is_zero).info()
+    sa, sb = rt.DTYPE_BYTES[type_a], rt.DTYPE_BYTES[type_b]
+    vecs = []
+    for va in (base["vec_a"], 1):
+        for vb in (base["vec_b"], 1):
+            if (va, vb) not in vecs:
+                vecs.append((va, vb))
+    tiles = {}
+    runs = [1 << k for k in range(0, 14)]
+    for va, vb in vecs:
+        for ra in runs:
+            for rb in runs:
+                if ra * rb > 4 * 16384:
+                    continue
+                cfg = {"vec_a": va, "vec_b": vb, "run_a": ra, "run_b": rb}
+                try:
+                    info = rt.Plan(type_a, type_b, perm, size, lda, ldb, beta_is_zero, cfg).info()
+                except rt.TTCError:
+                    continue
+                key = (tuple(info["tile"]), va, vb)
+                if key not in tiles:
+                    tiles[key] = (tile_score(info, sa, sb), {"tile": info["tile"], "vec_a": va, "vec_b": vb})
+    ranked = [c for _, c in sorted(tiles.values(), key=lambda x: -x[0])]
+    launch = []
+    for wg in (0, 2, 4):
+        for order in (1, 2):
+            for nt in ((1, 2) if beta_is_zero else (0,)):
+                launch.append({"wg_per_cu": wg, "grid_order": order, "nt_store": nt})
+    # sqrt pruning of both axes, as the reference combines its best blockings
+    # with its best loop orders
+    k = max(1, int(math.ceil(math.sqrt(max_impl))))
+    while len(ranked) * len(launch) > max_impl and len(ranked) > k:
+        ranked.pop()
+    while len(ranked) * len(launch) > max_impl and len(launch) > 1:
+        launch.pop()
+    out = []
+    for t in ranked:
+        for l in launch:
+            c = dict(t)
+            c.update(l)
+            out.append(c)
+    return out[:max(1, max_impl)]
+
+
+def measure(plan, A, B, alpha, beta, reps=5, flush=None, stream=None):
+    """Median kernel time in ms over `reps` launches (HIP events), the cache
+    flushed by rewriting `flush` (a >= 512 MiB device buffer) between them."""
+    import torch
+    times = []
+    plan.execute(A, B, alpha, beta, stream=stream)
+    for _ in range(reps):
+        if flush is not None:
+            flush.add_(1)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s = stream or torch.cuda.current_stream()
+        e0.record(s)
+        plan.execute(A, B, alpha, beta, stream=stream)
+        e1.record(s)
+        e1.synchronize()
+        times.append(e0.elapsed_time(e1))
+    times.sort()
+    return times[len(times) // 2]
+
+
+def _device_inputs(torch, type_a, type_b, perm, size, lda, ldb, dev):
+    r = len(size)
+    LA = list(lda) if lda else list(size)
+    LB = list(ldb) if ldb else [size[perm[q]] for q in range(r)]
+    na = int(math.prod(LA[:-1])) * size[-1] if r > 1 else size[0]
+    nb = int(math.prod(LB[:-1])) * size[perm[-1]] if r > 1 else size[0]
+    A = torch.empty(na * rt.DTYPE_BYTES[type_a], dtype=torch.uint8, device=dev).random_(0, 60)
+    B = torch.empty(nb * rt.DTYPE_BYTES[type_b], dtype=torch.uint8, device=dev).random_(0, 60)
+    return A, B
+
+
+def tune(type_a, type_b, perm, size, lda=None, ldb=None, alpha=1.0, beta=0.0, max_impl=200, reps=5,
+         progress=None):
+    """Times every candidate on device buffers of the problem's shape.
+    Returns results sorted fastest first: dicts with config, info, ms, GBs
+    (10^9 B/s) and GiBs (2^30 B/s, the reference's unit)."""
+    import torch
+    dev = torch.device("cuda:0")
+    bz = beta == 0
+    A, B = _device_inputs(torch, type_a, type_b, perm, size, lda, ldb, dev)
+    flush = torch.zeros((512 << 20) // 4, dtype=torch.float32, device=dev)
+    results = []
+    cands = candidates(type_a, type_b, perm, size, lda, ldb, bz, max_impl)
+    for i, cfg in enumerate(cands):
+        try:
+            plan = rt.Plan(type_a, type_b, perm, size, lda, ldb, bz, cfg)
+        except rt.TTCError:
+            continue
+        info = plan.info()
+        ms = measure(plan, A, B, alpha, beta, reps, flush)
+        nbytes = info["bytes_algorithmic"]
+        results.append({"config": cfg, "info": info, "ms": ms, "GBs": nbytes / (ms * 1e-3) / 1e9,
+                        "GiBs": nbytes / (ms * 1e-3) / float(1 << 30)})
+        if progress:
+            progress(i + 1, len(cands))
+    results.sort(key=lambda r: r["ms"])
+    return results

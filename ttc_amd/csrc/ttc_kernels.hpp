@@ -199,8 +199,8 @@ struct TileWorker {
   using TA = typename OP::TA;
   using TB = typename OP::TB;
   using L = typename OP::L;
-  static constexpr int NSA = kMaxElemsPerThread / VA;
-  static constexpr int NSB = kMaxElemsPerThread / VB;
+  static constexpr int NSA = slots_for(VA);
+  static constexpr int NSB = slots_for(VB);
   static constexpr bool FP8 = is_fp8<OP>::value;
 
   int32_t offA[NSA];

@@ -14,7 +14,12 @@ namespace ttc {
 
 constexpr int kMaxDim = 8;     // fused rank limit
 constexpr int kThreads = 256;  // work-group size (4 wave64)
-constexpr int kMaxElemsPerThread = 16;  // tile volume <= kThreads * 16
+constexpr int kMaxElemsPerThread = 16;
+
+// Vectors a thread owns per tile side for a vector width of `vec` elements:
+// 16 scalar elements, or 4 vectors once vectors are 4 elements or wider.  A
+// tile holds at most kThreads * slots_for(vec) * vec elements on each side.
+constexpr int slots_for(int vec) { return vec >= 4 ? 4 : kMaxElemsPerThread / vec; }
 
 // n / d for n < 2^31 via one multiply-high (round-up method): with
 // s = ceil(log2 d) and m = floor(2^32 (2^s - d) / d) + 1,

@@ -19,8 +19,6 @@ inline std::vector<int> vec_widths(int bytes) {
   if (wide > 2) w.push_back(wide);
   return w;
 }
-// Slots (vectors per thread per tile side) for a vector width.
-constexpr int slots_for(int vec) { return vec >= 4 ? 4 : kMaxElemsPerThread / vec; }
 
 int dtype_bytes(int t);
 // Bytes of the LDS staging type (alpha * a in A's arithmetic) for a type pair.
