@@ -112,7 +112,7 @@ def test_every_vector_width_and_mode(gpu):
 
 def test_explicit_tiles_with_two_partial_dims(gpu):
     perm, size = [1, 0, 2], [100, 90, 7]
-    for tile in ([32, 32, 1], [36, 28, 3], [100, 7, 2], [4, 90, 1], [20, 50, 7]):
+    for tile in ([32, 32, 1], [36, 28, 1], [100, 7, 2], [4, 90, 1], [20, 50, 4], [12, 8, 7]):
         check(gpu, S, S, perm, size, 1.0, 1.0, config=dict(tile=tile, vec_a=4 if tile[0] % 4 == 0 else 1,
                                                               vec_b=1 if tile[1] % 2 else 2))
 

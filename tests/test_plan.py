@@ -109,7 +109,10 @@ def _check_plan(i, perm, size):
     assert tiles == i["tiles"] or i["elements"] == 0
     assert i["tile"][0] % i["vec_a"] == 0 and i["tile"][i["perm"][0]] % i["vec_b"] == 0
     assert i["size"][0] % i["vec_a"] == 0 and i["size"][i["perm"][0]] % i["vec_b"] == 0
-    assert i["slots_a"] <= (16 if i["vec_a"] < 4 else 4) and i["slots_b"] <= (16 if i["vec_b"] < 4 else 4)
+    def slots(vec, nbytes):  # ttc_params.h slots_for
+        return min(16, max(4, 64 // (vec * nbytes)))
+    assert i["slots_a"] <= slots(i["vec_a"], rt.DTYPE_BYTES[i["type_a"]])
+    assert i["slots_b"] <= slots(i["vec_b"], rt.DTYPE_BYTES[i["type_b"]])
     assert 0 < i["lds_bytes"] <= 65536
     partial = sum(1 for k in range(r) if i["size"][k] % i["tile"][k])
     assert partial <= 2
@@ -158,6 +161,15 @@ def test_invalid_arguments_are_rejected():
         rt.Plan(S, S, [1, 0], [6, 4], config=dict(vec_a=4))   # 6 % 4 != 0
     with pytest.raises(rt.TTCError):
         rt.Plan(S, S, [1, 0], [100, 100], config=dict(tile=[100, 100]))  # volume above the kernel limit
+
+
+def test_empty_and_extent_one_problems_plan():
+    i = rt.Plan(S, S, [1, 0], [0, 5]).info()
+    assert i["elements"] == 0 and i["tiles"] == 0
+    perm = [15 - k for k in range(16)]
+    size = [2, 1, 2, 1, 2, 1, 2, 1, 1, 2, 1, 1, 1, 2, 1, 1]
+    i = rt.Plan(S, S, perm, size).info()  # extent-1 dimensions are dropped after fusion
+    assert i["rank"] <= 8 and i["elements"] == 64
 
 
 def test_too_many_partial_dims_is_unsupported():

@@ -73,24 +73,30 @@ def candidates(type_a, type_b, perm, size, lda=None, ldb=None, beta_is_zero=Fals
     return out[:max(1, max_impl)]
 
 
-def measure(plan, A, B, alpha, beta, reps=5, flush=None, stream=None):
-    """Median kernel time in ms over `reps` launches (HIP events), the cache
-    flushed by rewriting `flush` (a >= 512 MiB device buffer) between them."""
+def measure(plan, pairs, alpha, beta, reps=3, stream=None):
+    """Steady-state time per launch in ms: `reps` rounds of back-to-back
+    launches over the buffer pairs in `pairs`, timed with HIP events; the
+    median round.  Alternating between pairs whose total footprint exceeds
+    the last-level cache means every launch reads from and writes back to
+    HBM, dirty-line write-back of the previous launch included — as in a
+    stream of transpositions.  (A cache flush by writing a scratch buffer
+    right before a single timed launch would charge that flush's write-back
+    to the kernel.)"""
     import torch
-    times = []
-    plan.execute(A, B, alpha, beta, stream=stream)
-    for _ in range(reps):
-        if flush is not None:
-            flush.add_(1)
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        s = stream or torch.cuda.current_stream()
-        e0.record(s)
+    s = stream or torch.cuda.current_stream()
+    for A, B in pairs:
         plan.execute(A, B, alpha, beta, stream=stream)
+    rounds = []
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(s)
+        for A, B in pairs:
+            plan.execute(A, B, alpha, beta, stream=stream)
         e1.record(s)
         e1.synchronize()
-        times.append(e0.elapsed_time(e1))
-    times.sort()
-    return times[len(times) // 2]
+        rounds.append(e0.elapsed_time(e1) / len(pairs))
+    rounds.sort()
+    return rounds[len(rounds) // 2]
 
 
 def _device_inputs(torch, type_a, type_b, perm, size, lda, ldb, dev):
@@ -113,7 +119,11 @@ def tune(type_a, type_b, perm, size, lda=None, ldb=None, alpha=1.0, beta=0.0, ma
     dev = torch.device("cuda:0")
     bz = beta == 0
     A, B = _device_inputs(torch, type_a, type_b, perm, size, lda, ldb, dev)
-    flush = torch.zeros((512 << 20) // 4, dtype=torch.float32, device=dev)
+    pairs = [(A, B)]
+    # enough distinct buffers that one round does not fit the 256 MB cache
+    footprint = A.numel() + B.numel()
+    while footprint * len(pairs) < (768 << 20) and len(pairs) < 16:
+        pairs.append(_device_inputs(torch, type_a, type_b, perm, size, lda, ldb, dev))
     results = []
     cands = candidates(type_a, type_b, perm, size, lda, ldb, bz, max_impl)
     for i, cfg in enumerate(cands):
@@ -122,7 +132,7 @@ def tune(type_a, type_b, perm, size, lda=None, ldb=None, alpha=1.0, beta=0.0, ma
         except rt.TTCError:
             continue
         info = plan.info()
-        ms = measure(plan, A, B, alpha, beta, reps, flush)
+        ms = measure(plan, pairs, alpha, beta, reps)
         nbytes = info["bytes_algorithmic"]
         results.append({"config": cfg, "info": info, "ms": ms, "GBs": nbytes / (ms * 1e-3) / 1e9,
                         "GiBs": nbytes / (ms * 1e-3) / float(1 << 30)})

@@ -1,14 +1,17 @@
 // Batched grouped transpositions: many independent ops (uneven rank, size,
-// dtype, perm, alpha/beta) in ONE persistent launch driven by a device-side
-// work table, optionally replayed through a captured hipGraph.  The reference
-// has no counterpart (one transposition per host call,
-// ttc/CUDAtranspose.py:395-429).
+// dtype, perm, alpha/beta) driven by a device-side work table, optionally
+// replayed through a captured hipGraph.  The reference has no counterpart
+// (one transposition per host call, ttc/CUDAtranspose.py:395-429).
 //
-// Work table: one record per op (its tile parameters, pointers, scalars and a
-// kernel "kind"), the ops' tiles numbered consecutively, and a split of that
-// global tile range into one contiguous slice per work-group, balanced by
-// bytes moved.  A work-group walks its slice; when the slice crosses into the
-// next op it re-derives its per-thread tile offsets for that op.
+// Work table: one record per op (its tile parameters, pointers, scalars),
+// ops grouped by kernel kind (type pair, vector widths, beta read or not),
+// each group's tiles numbered consecutively, and a split of each group's tile
+// range into one contiguous slice per work-group, balanced by bytes moved.
+// One persistent launch per kind present walks that kind's slices; a
+// work-group whose slice crosses into the next op re-derives its per-thread
+// tile offsets for that op.  (One kernel for all kinds would carry the
+// register needs of its most demanding kind into every op: measured on
+// gfx950 it either ran at one wave per SIMD or spilled.)
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -29,46 +32,93 @@ struct BatchOpDev {
   double alpha, beta;
   int32_t kind;
   int32_t pad_;
-  uint32_t tile_begin;
+  uint32_t tile_begin;  // first tile of this op in its group
   uint32_t ntiles;
+  const TileBase* bases;  // this op's tile records (ntiles of them)
 };
+
+// Host version of the tile-origin decomposition (tiles numbered in A order
+// over the tile grid), filling one TileBase per tile.
+static void fill_bases(const TileParams& p, TileBase* out) {
+  for (uint32_t t = 0; t < p.ntiles; ++t) {
+    uint32_t rem = t;
+    TileBase b{0, 0, 0xffffu, 0xffffu};
+    for (int k = 0; k < p.rank; ++k) {
+      const uint32_t g = p.gdiv[k].d;
+      const uint32_t x = rem % g;
+      rem /= g;
+      b.a += (int64_t)x * p.tsa[k];
+      b.b += (int64_t)x * p.tsb[k];
+      const uint32_t left = (uint32_t)p.n[k] - x * (uint32_t)p.t[k];
+      if (k == p.split0) b.lim0 = left;
+      if (k == p.split1) b.lim1 = left;
+    }
+    out[t] = b;
+  }
+}
 
 // Kinds handled by the grouped kernel: same-type pairs, vector width 1 or the
 // widest on each side, beta read or not (non-temporal stores when not).
 // kind = ((pair * 2 + wide_a) * 2 + wide_b) * 2 + beta_is_zero
 enum BatchPair : int { kBS = 0, kBD = 1, kBBF16 = 2, kBF16 = 3, kBC = 4, kBZ = 5, kBPairs = 6 };
 
+__device__ __forceinline__ int32_t uni(int32_t x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ uint32_t uni(uint32_t x) { return (uint32_t)__builtin_amdgcn_readfirstlane((int32_t)x); }
+
+// The parameters the per-thread setup and the tile loop read, copied out of
+// the op record with each value marked wave-uniform, so that they live in
+// scalar registers as kernel arguments do in the single-op kernel.
+__device__ __forceinline__ void uniform_params(const TileParams& src, TileParams& p) {
+  p.rank = uni(src.rank);
+  p.V = uni(src.V);
+  p.split0 = uni(src.split0);
+  p.split1 = uni(src.split1);
+  p.tsplit0 = uni(src.tsplit0);
+  p.tsplit1 = uni(src.tsplit1);
+  p.splitB0 = uni(src.splitB0);
+  p.splitB1 = uni(src.splitB1);
+  p.lds_stride_b = uni(src.lds_stride_b);
+#pragma unroll
+  for (int k = 0; k < kMaxDim; ++k) {
+    p.t[k] = uni(src.t[k]);
+    p.sa[k] = uni(src.sa[k]);
+    p.ls[k] = uni(src.ls[k]);
+    p.tdiv[k].m = uni(src.tdiv[k].m);
+    p.tdiv[k].s = uni(src.tdiv[k].s);
+    p.tB[k] = uni(src.tB[k]);
+    p.sbB[k] = uni(src.sbB[k]);
+    p.lsB[k] = uni(src.lsB[k]);
+    p.tdivB[k].m = uni(src.tdivB[k].m);
+    p.tdivB[k].s = uni(src.tdivB[k].s);
+  }
+  const FastDiv* sd[6] = {&src.pA0, &src.pA1, &src.pB0, &src.pB1, &src.ts0, &src.ts1};
+  FastDiv* dd[6] = {&p.pA0, &p.pA1, &p.pB0, &p.pB1, &p.ts0, &p.ts1};
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    dd[i]->d = uni(sd[i]->d);
+    dd[i]->m = uni(sd[i]->m);
+    dd[i]->s = uni(sd[i]->s);
+  }
+}
+
 template <class OP, int VA, int VB, bool BETA>
 __device__ __forceinline__ void run_op_range(const BatchOpDev& o, uint32_t lo, uint32_t hi, unsigned char* lds_raw) {
   TileWorker<OP, VA, VB, BETA, !BETA> w;
-  w.setup(o.p, threadIdx.x);
+  TileParams up;
+  uniform_params(o.p, up);
+  w.setup(up, threadIdx.x);
   float amax = 0.0f;
   using L = typename OP::L;
   const typename OP::TA* A = static_cast<const typename OP::TA*>(o.A);
   typename OP::TB* B = static_cast<typename OP::TB*>(o.B);
   const typename OP::AL al = (typename OP::AL)o.alpha;
   const typename OP::BE be = (typename OP::BE)o.beta;
-  for (uint32_t t = lo; t < hi; ++t) w.tile(o.p, A, B, reinterpret_cast<L*>(lds_raw), t, al, be, 1.0f, amax);
+  w.run_table(up, A, B, reinterpret_cast<L*>(lds_raw), o.bases, lo, hi, al, be, 1.0f, amax);
 }
 
-template <class OP, int WA, int WB>
-__device__ __forceinline__ void run_op_beta(const BatchOpDev& o, uint32_t lo, uint32_t hi, int bz, unsigned char* l) {
-  if (bz) run_op_range<OP, WA, WB, false>(o, lo, hi, l);
-  else run_op_range<OP, WA, WB, true>(o, lo, hi, l);
-}
-
-template <class OP, int WIDE_A, int WIDE_B>
-__device__ __forceinline__ void run_op_pair(const BatchOpDev& o, uint32_t lo, uint32_t hi, int sub,
-                                            unsigned char* l) {
-  const int bz = sub & 1, wb = (sub >> 1) & 1, wa = (sub >> 2) & 1;
-  if (wa && wb) run_op_beta<OP, WIDE_A, WIDE_B>(o, lo, hi, bz, l);
-  else if (wa) run_op_beta<OP, WIDE_A, 1>(o, lo, hi, bz, l);
-  else if (wb) run_op_beta<OP, 1, WIDE_B>(o, lo, hi, bz, l);
-  else run_op_beta<OP, 1, 1>(o, lo, hi, bz, l);
-}
-
-__global__ void __launch_bounds__(kThreads) batch_kernel(const BatchOpDev* __restrict__ ops, int nops,
-                                                         const uint32_t* __restrict__ wg_begin) {
+template <class OP, int VA, int VB, bool BETA>
+__global__ void __launch_bounds__(kThreads) batch_group_kernel(const BatchOpDev* __restrict__ ops, int nops,
+                                                               const uint32_t* __restrict__ wg_begin) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   const uint32_t t0 = wg_begin[blockIdx.x], t1 = wg_begin[blockIdx.x + 1];
   if (t0 >= t1) return;
@@ -84,19 +134,42 @@ __global__ void __launch_bounds__(kThreads) batch_kernel(const BatchOpDev* __res
     const uint32_t end = o.tile_begin + o.ntiles;
     const uint32_t stop = end < t1 ? end : t1;
     if (t < stop) {
-      const uint32_t a = t - o.tile_begin, b = stop - o.tile_begin;
-      const int pair = o.kind >> 3, sub = o.kind & 7;
-      switch (pair) {
-        case kBS: run_op_pair<OpF32<float, float>, 4, 4>(o, a, b, sub, lds_raw); break;
-        case kBD: run_op_pair<OpDD, 2, 2>(o, a, b, sub, lds_raw); break;
-        case kBBF16: run_op_pair<OpF32<bf16s, bf16s>, 8, 8>(o, a, b, sub, lds_raw); break;
-        case kBF16: run_op_pair<OpF32<f16s, f16s>, 8, 8>(o, a, b, sub, lds_raw); break;
-        case kBC: run_op_pair<OpCC, 2, 2>(o, a, b, sub, lds_raw); break;
-        default: run_op_pair<OpZZ, 1, 1>(o, a, b, sub, lds_raw); break;
-      }
+      run_op_range<OP, VA, VB, BETA>(o, t - o.tile_begin, stop - o.tile_begin, lds_raw);
       t = stop;
     }
   }
+}
+
+// The kinds: kind = ((pair * 2 + wide_a) * 2 + wide_b) * 2 + beta_is_zero.
+template <class OP, int WA, int WB>
+static void add_pair(const void** tab, int pair) {
+  const void* k[8] = {
+      reinterpret_cast<const void*>(&batch_group_kernel<OP, 1, 1, true>),
+      reinterpret_cast<const void*>(&batch_group_kernel<OP, 1, 1, false>),
+      reinterpret_cast<const void*>(&batch_group_kernel<OP, 1, WB, true>),
+      reinterpret_cast<const void*>(&batch_group_kernel<OP, 1, WB, false>),
+      reinterpret_cast<const void*>(&batch_group_kernel<OP, WA, 1, true>),
+      reinterpret_cast<const void*>(&batch_group_kernel<OP, WA, 1, false>),
+      reinterpret_cast<const void*>(&batch_group_kernel<OP, WA, WB, true>),
+      reinterpret_cast<const void*>(&batch_group_kernel<OP, WA, WB, false>),
+  };
+  for (int i = 0; i < 8; ++i) tab[pair * 8 + i] = k[i];
+}
+
+struct GroupKernels {
+  const void* k[kBPairs * 8];
+  GroupKernels() {
+    add_pair<OpF32<float, float>, 4, 4>(k, kBS);
+    add_pair<OpDD, 2, 2>(k, kBD);
+    add_pair<OpF32<bf16s, bf16s>, 8, 8>(k, kBBF16);
+    add_pair<OpF32<f16s, f16s>, 8, 8>(k, kBF16);
+    add_pair<OpCC, 2, 2>(k, kBC);
+    add_pair<OpZZ, 1, 1>(k, kBZ);
+  }
+};
+static const void* group_kernel(int kind) {
+  static const GroupKernels tab;  // thread-safe initialisation
+  return tab.k[kind];
 }
 
 static int batch_pair(int ta, int tb) {
@@ -116,14 +189,23 @@ static int batch_pair(int ta, int tb) {
 
 using namespace ttc;
 
+// One launch of the grouped execution: the ops of one kind.
+struct BatchGroup {
+  int kind;
+  int op0, nops;   // slice of the device op table
+  int wg0, nwg;    // slice of the device work-group table (nwg + 1 entries)
+  int lds_bytes;
+};
+
 struct ttc_batch_s {
   int nops;
-  std::vector<ttc_plan_t> plans;      // per-op plans (also used by the per-op loop)
+  std::vector<ttc_plan_t> plans;  // per-op plans (also used by the per-op loop)
   std::vector<ttc_batch_op_t> ops;
+  std::vector<BatchGroup> groups;
   BatchOpDev* d_ops;
   uint32_t* d_wg;
-  int nwg;
-  int lds_bytes;
+  TileBase* d_bases;
+  int nwg_total;
   int64_t tiles, bytes;
   hipGraph_t graph[2];
   hipGraphExec_t exec[2];
@@ -134,11 +216,41 @@ static void batch_free(ttc_batch_s* b) {
     if (p) ttc_plan_destroy(p);
   if (b->d_ops) (void)hipFree(b->d_ops);
   if (b->d_wg) (void)hipFree(b->d_wg);
+  if (b->d_bases) (void)hipFree(b->d_bases);
   for (int i = 0; i < 2; ++i) {
     if (b->exec[i]) (void)hipGraphExecDestroy(b->exec[i]);
     if (b->graph[i]) (void)hipGraphDestroy(b->graph[i]);
   }
   delete b;
+}
+
+// Work-group slices of one group's tile range, balanced by bytes: returns
+// nwg + 1 boundaries (global tile indices of the group).
+static std::vector<uint32_t> split_by_bytes(const std::vector<BatchOpDev>& h, const std::vector<double>& weight,
+                                            int op0, int nops, uint32_t tiles, int nwg) {
+  std::vector<uint32_t> wgb(nwg + 1, tiles);
+  double total = 0;
+  for (int i = op0; i < op0 + nops; ++i) total += weight[i] * h[i].ntiles;
+  const double per = total / nwg;
+  int w = 1;
+  double acc = 0;
+  wgb[0] = 0;
+  for (int i = op0; i < op0 + nops && w < nwg; ++i) {
+    const double wt = weight[i];
+    const uint32_t n = h[i].ntiles;
+    if (n == 0) continue;
+    while (w < nwg && acc + wt * n >= per * w) {
+      const double need = per * w - acc;
+      const uint32_t k = wt > 0 ? (uint32_t)std::min<double>(n, std::max(0.0, std::ceil(need / wt))) : 0;
+      wgb[w] = h[i].tile_begin + k;
+      ++w;
+    }
+    acc += wt * n;
+  }
+  for (; w < nwg; ++w) wgb[w] = tiles;
+  wgb[nwg] = tiles;
+  for (int i = 1; i <= nwg; ++i) wgb[i] = std::max(wgb[i], wgb[i - 1]);
+  return wgb;
 }
 
 extern "C" ttc_status_t ttc_batch_create(ttc_batch_t* out, int nops, const ttc_batch_op_t* ops, int wg_per_cu) {
@@ -149,10 +261,8 @@ extern "C" ttc_status_t ttc_batch_create(ttc_batch_t* out, int nops, const ttc_b
   b->nops = nops;
   b->ops.assign(ops, ops + nops);
   b->plans.assign(nops, nullptr);
-  std::vector<BatchOpDev> h(nops);
-  std::vector<double> weight(nops);
-  uint64_t tiles = 0;
-  int lds_max = 16;
+  std::vector<BatchOpDev> rec(nops);
+  std::vector<int> kinds(nops);
   b->bytes = 0;
   for (int i = 0; i < nops; ++i) {
     const ttc_batch_op_t& op = ops[i];
@@ -170,8 +280,7 @@ extern "C" ttc_status_t ttc_batch_create(ttc_batch_t* out, int nops, const ttc_b
       any_lda |= op.lda[k] != 0;
       any_ldb |= op.ldb[k] != 0;
     }
-    // vector widths: 1 or the widest of the type (the kinds the grouped
-    // kernel instantiates)
+    // vector widths: 1 or the widest of the type (the kinds instantiated)
     ttc_config_t cfg = op.cfg;
     const int wide = std::max(1, 16 / dtype_bytes(op.type_a));
     for (int pass = 0; pass < 2; ++pass) {
@@ -199,12 +308,16 @@ extern "C" ttc_status_t ttc_batch_create(ttc_batch_t* out, int nops, const ttc_b
       batch_free(b);
       return TTC_ERROR_UNSUPPORTED;
     }
-    const ttc_plan_s* use = variant_for(pl, op.A, op.B);
-    if (!use || (pl->g.elements > 0 && (!op.A || !op.B))) {
+    if (pl->g.elements > 0 && (!op.A || !op.B)) {
       batch_free(b);
       return TTC_ERROR_INVALID_VALUE;
     }
-    BatchOpDev& d = h[i];
+    const ttc_plan_s* use = pl->g.elements > 0 ? variant_for(pl, op.A, op.B) : pl;
+    if (!use) {
+      batch_free(b);
+      return TTC_ERROR_INVALID_VALUE;
+    }
+    BatchOpDev& d = rec[i];
     std::memset(&d, 0, sizeof(d));
     d.p = use->p;
     d.A = op.A;
@@ -213,62 +326,84 @@ extern "C" ttc_status_t ttc_batch_create(ttc_batch_t* out, int nops, const ttc_b
     d.beta = op.beta;
     const int wa = use->va > 1 ? 1 : 0, wb = use->vb > 1 ? 1 : 0;
     d.kind = ((pair * 2 + wa) * 2 + wb) * 2 + (pl->beta_is_zero ? 1 : 0);
-    d.tile_begin = (uint32_t)tiles;
     d.ntiles = pl->g.elements > 0 ? use->p.ntiles : 0;
-    tiles += d.ntiles;
+    kinds[i] = d.kind;
+    const int64_t sa = dtype_bytes(op.type_a), sb = dtype_bytes(op.type_b);
+    b->bytes += pl->g.elements * (sa + sb * (pl->beta_is_zero ? 1 : 2));
+  }
+  // order the table by kind (stable: ops keep their order inside a kind)
+  std::vector<int> order(nops);
+  for (int i = 0; i < nops; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return kinds[x] < kinds[y]; });
+  std::vector<BatchOpDev> h(nops);
+  std::vector<double> weight(nops);
+  for (int j = 0; j < nops; ++j) h[j] = rec[order[j]];
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  const int cus = device_cus(dev);
+  std::vector<uint32_t> wg_all;
+  b->tiles = 0;
+  for (int j = 0; j < nops;) {
+    int e = j;
+    uint64_t tiles = 0;
+    int lds = 16;
+    while (e < nops && h[e].kind == h[j].kind) {
+      h[e].tile_begin = (uint32_t)tiles;
+      tiles += h[e].ntiles;
+      lds = std::max(lds, h[e].p.lds_elems * lds_elem_bytes(ops[order[e]].type_a, ops[order[e]].type_b));
+      const double bytes = (double)(dtype_bytes(ops[order[e]].type_a) +
+                                    dtype_bytes(ops[order[e]].type_b) * ((h[e].kind & 1) ? 1 : 2));
+      weight[e] = h[e].ntiles ? bytes * (double)h[e].p.V : 0.0;
+      ++e;
+    }
     if (tiles >= (1ull << 31)) {
       batch_free(b);
       return TTC_ERROR_UNSUPPORTED;
     }
-    lds_max = std::max(lds_max, use->lds_bytes);
-    const int64_t sa = dtype_bytes(op.type_a), sb = dtype_bytes(op.type_b);
-    const int64_t bytes = pl->g.elements * (sa + sb * (pl->beta_is_zero ? 1 : 2));
-    b->bytes += bytes;
-    weight[i] = d.ntiles ? (double)bytes / d.ntiles : 0.0;
-  }
-  b->tiles = (int64_t)tiles;
-  b->lds_bytes = lds_max;
-  // work-groups: persistent, as many as fit on the device
-  int dev = 0;
-  (void)hipGetDevice(&dev);
-  const int cus = device_cus(dev);
-  int wg = wg_per_cu > 0 ? wg_per_cu : occupancy(reinterpret_cast<const void*>(&batch_kernel), lds_max);
-  if (wg < 1) wg = 1;
-  int nwg = (int)std::min<uint64_t>((uint64_t)cus * wg, std::max<uint64_t>(tiles, 1));
-  b->nwg = nwg;
-  // slice the global tile range by bytes
-  std::vector<uint32_t> wgb(nwg + 1, (uint32_t)tiles);
-  {
-    double total = 0;
-    for (int i = 0; i < nops; ++i) total += weight[i] * h[i].ntiles;
-    const double per = total / nwg;
-    int w = 1;
-    double acc = 0;
-    wgb[0] = 0;
-    for (int i = 0; i < nops && w < nwg; ++i) {
-      const double wt = weight[i];
-      const uint32_t n = h[i].ntiles;
-      if (n == 0) continue;
-      // tiles of op i where the cumulative byte count crosses w * per
-      while (w < nwg && acc + wt * n >= per * w) {
-        double need = per * w - acc;
-        uint32_t k = wt > 0 ? (uint32_t)std::min<double>(n, std::max(0.0, std::ceil(need / wt))) : 0;
-        wgb[w] = h[i].tile_begin + k;
-        ++w;
-      }
-      acc += wt * n;
+    lds = (lds + 15) / 16 * 16;
+    if (tiles > 0) {
+      BatchGroup grp;
+      grp.kind = h[j].kind;
+      grp.op0 = j;
+      grp.nops = e - j;
+      grp.lds_bytes = lds;
+      const void* fn = group_kernel(grp.kind);
+      const int occ = occupancy(fn, lds);
+      int wg = wg_per_cu > 0 ? wg_per_cu : std::min(occ, 4);
+      if (wg < 1) wg = 1;
+      grp.nwg = (int)std::min<uint64_t>((uint64_t)cus * wg, tiles);
+      grp.wg0 = (int)wg_all.size();
+      std::vector<uint32_t> wgb = split_by_bytes(h, weight, j, e - j, (uint32_t)tiles, grp.nwg);
+      wg_all.insert(wg_all.end(), wgb.begin(), wgb.end());
+      b->groups.push_back(grp);
+      b->tiles += (int64_t)tiles;
     }
-    for (; w < nwg; ++w) wgb[w] = (uint32_t)tiles;
-    wgb[nwg] = (uint32_t)tiles;
-    for (int i = 1; i <= nwg; ++i) wgb[i] = std::max(wgb[i], wgb[i - 1]);
+    j = e;
+  }
+  b->nwg_total = 0;
+  for (const BatchGroup& g : b->groups) b->nwg_total += g.nwg;
+  // per-tile records of every op, in table order
+  std::vector<TileBase> bases((size_t)std::max<int64_t>(b->tiles, 1));
+  std::vector<size_t> base_off(nops);
+  {
+    size_t off = 0;
+    for (int j = 0; j < nops; ++j) {
+      base_off[j] = off;
+      if (h[j].ntiles) fill_bases(h[j].p, bases.data() + off);
+      off += h[j].ntiles;
+    }
   }
   if (hipMalloc(&b->d_ops, sizeof(BatchOpDev) * nops) != hipSuccess ||
-      hipMalloc(&b->d_wg, sizeof(uint32_t) * (nwg + 1)) != hipSuccess) {
+      hipMalloc(&b->d_wg, sizeof(uint32_t) * std::max<size_t>(wg_all.size(), 1)) != hipSuccess ||
+      hipMalloc(&b->d_bases, sizeof(TileBase) * bases.size()) != hipSuccess) {
     batch_free(b);
     return TTC_ERROR_ALLOC;
   }
+  for (int j = 0; j < nops; ++j) h[j].bases = b->d_bases + base_off[j];
   if (hipMemcpy(b->d_ops, h.data(), sizeof(BatchOpDev) * nops, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(b->d_wg, wgb.data(), sizeof(uint32_t) * (nwg + 1), hipMemcpyHostToDevice) != hipSuccess) {
+      (!wg_all.empty() && hipMemcpy(b->d_wg, wg_all.data(), sizeof(uint32_t) * wg_all.size(),
+                                    hipMemcpyHostToDevice) != hipSuccess) ||
+      hipMemcpy(b->d_bases, bases.data(), sizeof(TileBase) * bases.size(), hipMemcpyHostToDevice) != hipSuccess) {
     batch_free(b);
     return TTC_ERROR_HIP;
   }
@@ -278,14 +413,16 @@ extern "C" ttc_status_t ttc_batch_create(ttc_batch_t* out, int nops, const ttc_b
 
 extern "C" ttc_status_t ttc_batch_execute(ttc_batch_t b, ttc_stream_t stream) {
   if (!b) return TTC_ERROR_INVALID_VALUE;
-  if (b->tiles == 0) return TTC_SUCCESS;
-  const BatchOpDev* ops = b->d_ops;
-  int nops = b->nops;
-  const uint32_t* wg = b->d_wg;
-  void* args[] = {&ops, &nops, &wg};
-  hipError_t e = hipLaunchKernel(reinterpret_cast<const void*>(&batch_kernel), dim3(b->nwg), dim3(kThreads), args,
-                                 (size_t)b->lds_bytes, static_cast<hipStream_t>(stream));
-  return e == hipSuccess ? TTC_SUCCESS : TTC_ERROR_HIP;
+  for (const BatchGroup& g : b->groups) {
+    const BatchOpDev* ops = b->d_ops + g.op0;
+    int nops = g.nops;
+    const uint32_t* wg = b->d_wg + g.wg0;
+    void* args[] = {&ops, &nops, &wg};
+    hipError_t e = hipLaunchKernel(group_kernel(g.kind), dim3(g.nwg), dim3(kThreads), args, (size_t)g.lds_bytes,
+                                   static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return TTC_ERROR_HIP;
+  }
+  return TTC_SUCCESS;
 }
 
 extern "C" ttc_status_t ttc_batch_execute_loop(ttc_batch_t b, ttc_stream_t stream) {
@@ -325,7 +462,7 @@ extern "C" ttc_status_t ttc_batch_graph_launch(ttc_batch_t b, ttc_stream_t strea
 extern "C" ttc_status_t ttc_batch_get_info(ttc_batch_t b, int64_t* tiles, int* workgroups, int64_t* bytes) {
   if (!b) return TTC_ERROR_INVALID_VALUE;
   if (tiles) *tiles = b->tiles;
-  if (workgroups) *workgroups = b->nwg;
+  if (workgroups) *workgroups = b->nwg_total;
   if (bytes) *bytes = b->bytes;
   return TTC_SUCCESS;
 }

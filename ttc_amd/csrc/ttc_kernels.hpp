@@ -190,6 +190,14 @@ template <class T, int N, bool NT> __device__ __forceinline__ void vstore(T* p, 
   else *reinterpret_cast<R*>(p) = r;
 }
 
+// Where one tile of an op sits (host-precomputed for grouped launches):
+// element offsets of its origin in A and B and its extent along the split
+// dimensions (0xffff when the op has no such dimension).
+struct TileBase {
+  int64_t a, b;
+  uint32_t lim0, lim1;
+};
+
 // ------------------------------------------------------------- the tile
 // VA / VB: elements per lane vector on the read / write side.  Vectors never
 // cross a tile row: the planner guarantees t[0] % VA == 0 and
@@ -199,21 +207,28 @@ struct TileWorker {
   using TA = typename OP::TA;
   using TB = typename OP::TB;
   using L = typename OP::L;
-  static constexpr int NSA = slots_for(VA);
-  static constexpr int NSB = slots_for(VB);
+  static constexpr int NSA = slots_for(VA, (int)sizeof(TA));
+  static constexpr int NSB = slots_for(VB, (int)sizeof(TB));
   static constexpr bool FP8 = is_fp8<OP>::value;
 
-  int32_t offA[NSA];
-  int32_t ldsA[NSA];
-  uint32_t crdA[NSA];
-  int32_t offB[NSB];
-  int32_t ldsB[NSB];
-  uint32_t crdB[NSB];
+  // Per slot: the tile-relative byte offset in A (resp. B) — unsigned 32-bit,
+  // so that loads and stores use the scalar-base + 32-bit vector-offset
+  // address form — and the LDS index, two 16-bit indices per register (LDS
+  // holds < 65536 elements).
+  uint32_t offA[NSA];
+  uint32_t ldsA2[(NSA + 1) / 2];
+  uint32_t offB[NSB];
+  uint32_t ldsB2[(NSB + 1) / 2];
   int nA, nB;  // slots this thread owns (the last may be partially used)
+  int tid;
+
+  __device__ __forceinline__ int lds_a(int s) const { return (int)((ldsA2[s >> 1] >> ((s & 1) * 16)) & 0xffffu); }
+  __device__ __forceinline__ int lds_b(int s) const { return (int)((ldsB2[s >> 1] >> ((s & 1) * 16)) & 0xffffu); }
 
   // Per-thread, once per launch (or per op in a grouped launch): decompose
   // this thread's vector positions inside a tile.
-  __device__ __forceinline__ void setup(const TileParams& p, int tid) {
+  __device__ __forceinline__ void setup(const TileParams& p, int tid_) {
+    tid = tid_;
     const int nvA = p.V / VA;
     const int nvB = p.V / VB;
     nA = (nvA - tid + kThreads - 1) / kThreads;
@@ -222,7 +237,6 @@ struct TileWorker {
     for (int s = 0; s < NSA; ++s) {
       uint32_t l = (uint32_t)(s * kThreads + tid) * VA;
       int32_t oa = 0, ol = 0;
-      uint32_t c = 0;
 #pragma unroll
       for (int k = 0; k < kMaxDim; ++k) {
         if (k < p.rank) {
@@ -231,101 +245,192 @@ struct TileWorker {
           l = q;
           oa += (int32_t)j * p.sa[k];
           ol += (int32_t)j * p.ls[k];
-          if (k == p.split0) c |= j;
-          if (k == p.split1) c |= j << 16;
         }
       }
-      offA[s] = oa; ldsA[s] = ol; crdA[s] = c;
+      offA[s] = (uint32_t)oa * (uint32_t)sizeof(TA);
+      if (s & 1) ldsA2[s >> 1] |= (uint32_t)ol << 16;
+      else ldsA2[s >> 1] = (uint32_t)ol;
     }
 #pragma unroll
     for (int s = 0; s < NSB; ++s) {
       uint32_t l = (uint32_t)(s * kThreads + tid) * VB;
       int32_t ob = 0, ol = 0;
-      uint32_t c = 0;
 #pragma unroll
       for (int q = 0; q < kMaxDim; ++q) {
         if (q < p.rank) {
-          const int k = p.perm[q];
-          const uint32_t qq = fdiv(l, p.tdiv[k]);
-          const uint32_t j = l - qq * (uint32_t)p.t[k];
+          const uint32_t qq = fdiv(l, p.tdivB[q]);
+          const uint32_t j = l - qq * (uint32_t)p.tB[q];
           l = qq;
-          ob += (int32_t)j * p.sb[k];
-          ol += (int32_t)j * p.ls[k];
-          if (k == p.split0) c |= j;
-          if (k == p.split1) c |= j << 16;
+          ob += (int32_t)j * p.sbB[q];
+          ol += (int32_t)j * p.lsB[q];
         }
       }
-      offB[s] = ob; ldsB[s] = ol; crdB[s] = c;
+      offB[s] = (uint32_t)ob * (uint32_t)sizeof(TB);
+      if (s & 1) ldsB2[s >> 1] |= (uint32_t)ol << 16;
+      else ldsB2[s >> 1] = (uint32_t)ol;
     }
   }
 
-  // One tile.  lim0/lim1: extent of this tile along the split dims.
-  __device__ __forceinline__ void tile(const TileParams& p, const TA* __restrict__ A, TB* __restrict__ B,
-                                       L* __restrict__ lds, uint32_t tile_id, typename OP::AL alpha,
-                                       typename OP::BE beta, float scale, float& amax) {
+  // Where a tile sits: base offsets in A and B and its extent along the
+  // split dimensions (lim0/lim1; 0xffff when there is no such dimension).
+  struct Coord {
+    int64_t bA, bB;
+    uint32_t lim0, lim1;
+    bool full;
+  };
+
+  __device__ __forceinline__ Coord coord(const TileParams& p, uint32_t tile_id) const {
+    Coord c;
     uint32_t rem = tile_id;
-    int64_t bA = 0, bB = 0;
-    uint32_t lim0 = 0xffffu, lim1 = 0xffffu;
+    c.bA = 0;
+    c.bB = 0;
+    c.lim0 = 0xffffu;
+    c.lim1 = 0xffffu;
 #pragma unroll
     for (int k = 0; k < kMaxDim; ++k) {
       if (k < p.rank) {
         const uint32_t q = fdiv(rem, p.gdiv[k]);
-        const uint32_t c = rem - q * p.gdiv[k].d;
+        const uint32_t x = rem - q * p.gdiv[k].d;
         rem = q;
-        bA += (int64_t)c * p.tsa[k];
-        bB += (int64_t)c * p.tsb[k];
-        const uint32_t left = (uint32_t)p.n[k] - c * (uint32_t)p.t[k];
-        if (k == p.split0) lim0 = left;
-        if (k == p.split1) lim1 = left;
+        c.bA += (int64_t)x * p.tsa[k];
+        c.bB += (int64_t)x * p.tsb[k];
+        const uint32_t left = (uint32_t)p.n[k] - x * (uint32_t)p.t[k];
+        if (k == p.split0) c.lim0 = left;
+        if (k == p.split1) c.lim1 = left;
       }
     }
-    const bool full = lim0 >= (uint32_t)p.tsplit0 && lim1 >= (uint32_t)p.tsplit1;
-    const TA* a = A + bA;
-    TB* b = B + bB;
+    c.full = c.lim0 >= (uint32_t)p.tsplit0 && c.lim1 >= (uint32_t)p.tsplit1;
+    return c;
+  }
 
-    // read phase: global -> registers -> (alpha) -> LDS
-    TA va[NSA][VA];
+  // Digit (l / P) mod t of a split dimension in tile-linear index l.
+  static __device__ __forceinline__ uint32_t digit(uint32_t l, const FastDiv& P, const FastDiv& t) {
+    const uint32_t x = fdiv(l, P);
+    return x - fdiv(x, t) * t.d;
+  }
+  // Whether slot s holds data of the tile at c.  Full tiles need no test;
+  // in an edge tile the slot's coordinates along the split dimensions are
+  // recomputed from its tile-linear index (cheaper than keeping them live).
+  __device__ __forceinline__ bool in_a(const TileParams& p, const Coord& c, int s) const {
+    if (s >= nA) return false;
+    if (c.full) return true;
+    uint32_t l = (uint32_t)(s * kThreads + tid) * VA;
+    asm volatile("" : "+v"(l));  // recompute per edge tile; do not hoist out of the tile loop
+    return (p.split0 < 0 || digit(l, p.pA0, p.ts0) < c.lim0) && (p.split1 < 0 || digit(l, p.pA1, p.ts1) < c.lim1);
+  }
+  __device__ __forceinline__ bool in_b(const TileParams& p, const Coord& c, int s) const {
+    if (s >= nB) return false;
+    if (c.full) return true;
+    uint32_t l = (uint32_t)(s * kThreads + tid) * VB;
+    asm volatile("" : "+v"(l));  // recompute per edge tile; do not hoist out of the tile loop
+    return (p.split0 < 0 || digit(l, p.pB0, p.ts0) < c.lim0) && (p.split1 < 0 || digit(l, p.pB1, p.ts1) < c.lim1);
+  }
+
+  // Global -> registers for the tile at c.
+  __device__ __forceinline__ void load(const TileParams& p, const Coord& c, const TA* __restrict__ A,
+                                       TA (&va)[NSA][VA]) const {
+    const char* a = reinterpret_cast<const char*>(A + c.bA);
+#pragma unroll
+    for (int s = 0; s < NSA; ++s)
+      if (in_a(p, c, s)) vload<TA, VA>(va[s], reinterpret_cast<const TA*>(a + offA[s]));
+  }
+
+  // Registers -> (alpha) -> LDS.
+  __device__ __forceinline__ void stage(const TileParams& p, const Coord& c, const TA (&va)[NSA][VA],
+                                        L* __restrict__ lds, typename OP::AL alpha, float& amax) const {
 #pragma unroll
     for (int s = 0; s < NSA; ++s) {
-      const bool ok = s < nA && (full || ((crdA[s] & 0xffffu) < lim0 && (crdA[s] >> 16) < lim1));
-      if (ok) vload<TA, VA>(va[s], a + offA[s]);
-    }
-#pragma unroll
-    for (int s = 0; s < NSA; ++s) {
-      const bool ok = s < nA && (full || ((crdA[s] & 0xffffu) < lim0 && (crdA[s] >> 16) < lim1));
-      if (ok) {
+      if (in_a(p, c, s)) {
+        const int li = lds_a(s);
 #pragma unroll
         for (int v = 0; v < VA; ++v) {
           const L x = OP::prod(alpha, va[s][v]);
           if constexpr (FP8) amax = fmaxf(amax, fabsf(x));
-          lds[ldsA[s] + v] = x;
+          lds[li + v] = x;
         }
       }
     }
-    __syncthreads();
-    // write phase: LDS -> (beta epilogue) -> global
+  }
+
+  // LDS -> (beta epilogue) -> global.
+  __device__ __forceinline__ void store(const TileParams& p, const Coord& c, TB* __restrict__ B,
+                                        const L* __restrict__ lds, typename OP::BE beta, float scale) const {
+    char* b = reinterpret_cast<char*>(B + c.bB);
     const int S = p.lds_stride_b;
 #pragma unroll
     for (int s = 0; s < NSB; ++s) {
-      const bool ok = s < nB && (full || ((crdB[s] & 0xffffu) < lim0 && (crdB[s] >> 16) < lim1));
-      if (ok) {
+      if (in_b(p, c, s)) {
+        const int li = lds_b(s);
         TB o[VB];
         if constexpr (FP8) {
 #pragma unroll
-          for (int v = 0; v < VB; ++v) o[v] = OP::outs(lds[ldsB[s] + v * S], scale);
+          for (int v = 0; v < VB; ++v) o[v] = OP::outs(lds[li + v * S], scale);
         } else if constexpr (BETA) {
           TB ob[VB];
-          vload<TB, VB>(ob, b + offB[s]);
+          vload<TB, VB>(ob, reinterpret_cast<const TB*>(b + offB[s]));
 #pragma unroll
-          for (int v = 0; v < VB; ++v) o[v] = OP::outb(lds[ldsB[s] + v * S], beta, ob[v]);
+          for (int v = 0; v < VB; ++v) o[v] = OP::outb(lds[li + v * S], beta, ob[v]);
         } else {
 #pragma unroll
-          for (int v = 0; v < VB; ++v) o[v] = OP::out(lds[ldsB[s] + v * S]);
+          for (int v = 0; v < VB; ++v) o[v] = OP::out(lds[li + v * S]);
         }
-        vstore<TB, VB, NT>(b + offB[s], o);
+        vstore<TB, VB, NT>(reinterpret_cast<TB*>(b + offB[s]), o);
       }
     }
-    __syncthreads();
+  }
+
+  // The same walk over tiles whose Coord comes from a table (one record per
+  // tile, precomputed on the host) instead of being derived from `p`.
+  __device__ __forceinline__ Coord coord_of(const TileParams& p, const TileBase& b) const {
+    Coord c;
+    c.bA = b.a;
+    c.bB = b.b;
+    c.lim0 = b.lim0;
+    c.lim1 = b.lim1;
+    c.full = c.lim0 >= (uint32_t)p.tsplit0 && c.lim1 >= (uint32_t)p.tsplit1;
+    return c;
+  }
+  __device__ __forceinline__ void run_table(const TileParams& p, const TA* __restrict__ A, TB* __restrict__ B,
+                                            L* __restrict__ lds, const TileBase* __restrict__ bases, uint32_t first,
+                                            uint32_t end, typename OP::AL alpha, typename OP::BE beta, float scale,
+                                            float& amax) {
+    if (first >= end) return;
+    TA va[NSA][VA];
+    Coord c = coord_of(p, bases[first]);
+    load(p, c, A, va);
+    for (uint32_t t = first; t < end; ++t) {
+      stage(p, c, va, lds, alpha, amax);
+      __syncthreads();
+      const bool more = t + 1 < end;
+      const Coord cn = coord_of(p, bases[more ? t + 1 : t]);
+      if (more) load(p, cn, A, va);
+      store(p, c, B, lds, beta, scale);
+      __syncthreads();
+      c = cn;
+    }
+  }
+
+  // All tiles first, first+step, ... < end.  Software pipelined: the loads
+  // of the next tile are issued before the write phase of the current one,
+  // so each work-group keeps a tile of reads in flight while it writes.
+  __device__ __forceinline__ void run(const TileParams& p, const TA* __restrict__ A, TB* __restrict__ B,
+                                      L* __restrict__ lds, uint32_t first, uint32_t end, uint32_t step,
+                                      typename OP::AL alpha, typename OP::BE beta, float scale, float& amax) {
+    if (first >= end) return;
+    TA va[NSA][VA];
+    Coord c = coord(p, first);
+    load(p, c, A, va);
+    for (uint32_t t = first; t < end; t += step) {
+      stage(p, c, va, lds, alpha, amax);
+      __syncthreads();
+      const uint32_t tn = t + step;
+      const bool more = tn < end;
+      const Coord cn = coord(p, more ? tn : t);
+      if (more) load(p, cn, A, va);
+      store(p, c, B, lds, beta, scale);
+      __syncthreads();
+      c = cn;
+    }
   }
 };
 
@@ -376,9 +481,7 @@ __global__ void __launch_bounds__(kThreads) transpose_kernel(const TileParams p,
   w.setup(p, threadIdx.x);
   const TileRange r = tile_range(p.ntiles, blockIdx.x, gridDim.x, p.grid_order);
   float amax = 0.0f;
-  const typename OP::AL al = (typename OP::AL)alpha;
-  const typename OP::BE be = (typename OP::BE)beta;
-  for (uint32_t t = r.first; t < r.end; t += r.step) w.tile(p, A, B, lds, t, al, be, 1.0f, amax);
+  w.run(p, A, B, lds, r.first, r.end, r.step, (typename OP::AL)alpha, (typename OP::BE)beta, 1.0f, amax);
 }
 
 // fp8 cast-transpose.  The scale comes from *scale_in (delayed scaling),
@@ -405,7 +508,7 @@ __global__ void __launch_bounds__(kThreads) cast_fp8_kernel(const TileParams p, 
   w.setup(p, threadIdx.x);
   const TileRange r = tile_range(p.ntiles, blockIdx.x, gridDim.x, p.grid_order);
   float amax = 0.0f;
-  for (uint32_t t = r.first; t < r.end; t += r.step) w.tile(p, A, B, lds, t, alpha, 0.0f, scale, amax);
+  w.run(p, A, B, lds, r.first, r.end, r.step, alpha, 0.0f, scale, amax);
   if (amax_out) block_amax_commit(amax, amax_out);
 }
 

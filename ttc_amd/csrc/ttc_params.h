@@ -14,12 +14,14 @@ namespace ttc {
 
 constexpr int kMaxDim = 8;     // fused rank limit
 constexpr int kThreads = 256;  // work-group size (4 wave64)
-constexpr int kMaxElemsPerThread = 16;
 
-// Vectors a thread owns per tile side for a vector width of `vec` elements:
-// 16 scalar elements, or 4 vectors once vectors are 4 elements or wider.  A
-// tile holds at most kThreads * slots_for(vec) * vec elements on each side.
-constexpr int slots_for(int vec) { return vec >= 4 ? 4 : kMaxElemsPerThread / vec; }
+// Vectors a thread owns per tile side for vectors of `vec` elements of
+// `bytes` bytes: as many as fit 64 bytes of data (16 VGPRs), between 4 and
+// 16.  A tile holds at most kThreads * slots_for(vec, bytes) * vec elements
+// on each side; planner and kernels use this one rule.
+constexpr int slots_for(int vec, int bytes) {
+  return 64 / (vec * bytes) > 16 ? 16 : (64 / (vec * bytes) < 4 ? 4 : 64 / (vec * bytes));
+}
 
 // n / d for n < 2^31 via one multiply-high (round-up method): with
 // s = ceil(log2 d) and m = floor(2^32 (2^s - d) / d) + 1,
@@ -51,6 +53,18 @@ struct TileParams {
   int64_t tsb[kMaxDim];        // t[k] * B stride
   FastDiv tdiv[kMaxDim];       // division by t[k]
   FastDiv gdiv[kMaxDim];       // division by g[k] (tiles along dim k)
+  // The same quantities in B order (position q describes A dimension
+  // perm[q]), so that device code never indexes this struct with a value
+  // known only at run time (that would force a copy into scratch memory).
+  FastDiv tdivB[kMaxDim];
+  int32_t tB[kMaxDim];
+  int32_t sbB[kMaxDim];
+  int32_t lsB[kMaxDim];
+  int32_t splitB0, splitB1;    // B positions of split0 / split1 (-1 = none)
+  // Digit of a split dimension in a tile-linear index l (A or B order):
+  // (l / P) mod t, P = product of the tile extents before it in that order.
+  FastDiv pA0, pA1, pB0, pB1;  // division by P
+  FastDiv ts0, ts1;            // division by t[split]
   uint32_t ntiles;
   int32_t split0, split1;      // dims with partial tiles (-1 = none)
   int32_t tsplit0, tsplit1;    // t[split] (0 when none)

@@ -120,30 +120,48 @@ ttc_status_t make_geometry(int dim, const int* perm, const int* size, const int*
   int fperm[TTC_MAX_INPUT_DIM], fsize[TTC_MAX_INPUT_DIM], flda[TTC_MAX_INPUT_DIM], fldb[TTC_MAX_INPUT_DIM];
   ttc_status_t st = fuse_indices(dim, perm, size, lda, ldb, &r, fperm, fsize, flda, fldb);
   if (st != TTC_SUCCESS) return st;
-  if (r > kMaxDim) return TTC_ERROR_UNSUPPORTED;
-  g = Geometry();
-  g.rank = r;
-  g.elements = 1;
-  for (int k = 0; k < r; ++k) {
-    g.perm[k] = fperm[k];
-    g.size[k] = fsize[k];
-    g.lda[k] = flda[k];
-    g.ldb[k] = fldb[k];
-    g.elements *= fsize[k];
-  }
-  int64_t sbpos[kMaxDim];
-  g.sa[0] = 1;
+  // strides of the fused problem
+  int64_t sa[TTC_MAX_INPUT_DIM], sb[TTC_MAX_INPUT_DIM], sbpos[TTC_MAX_INPUT_DIM];
+  sa[0] = 1;
   sbpos[0] = 1;
   for (int k = 1; k < r; ++k) {
-    g.sa[k] = g.sa[k - 1] * flda[k - 1];
+    sa[k] = sa[k - 1] * flda[k - 1];
     sbpos[k] = sbpos[k - 1] * fldb[k - 1];
   }
-  for (int q = 0; q < r; ++q) g.sb[fperm[q]] = sbpos[q];
+  for (int q = 0; q < r; ++q) sb[fperm[q]] = sbpos[q];
+  // Extent-1 dimensions index nothing: drop them (their padding is already
+  // in the strides of the others), except A's and B's stride-1 dimensions.
+  int keep[TTC_MAX_INPUT_DIM], newid[TTC_MAX_INPUT_DIM], nk = 0;
+  for (int k = 0; k < r; ++k) {
+    keep[k] = fsize[k] != 1 || k == 0 || k == fperm[0];
+    newid[k] = keep[k] ? nk++ : -1;
+  }
+  if (nk > kMaxDim) return TTC_ERROR_UNSUPPORTED;
+  g = Geometry();
+  g.rank = nk;
+  g.elements = 1;
+  for (int k = 0; k < r; ++k) {
+    g.elements *= fsize[k];
+    if (!keep[k]) continue;
+    const int n = newid[k];
+    g.size[n] = fsize[k];
+    g.lda[n] = flda[k];
+    g.sa[n] = sa[k];
+    g.sb[n] = sb[k];
+  }
+  int q2 = 0;
+  for (int q = 0; q < r; ++q) {
+    if (!keep[fperm[q]]) continue;
+    g.perm[q2] = newid[fperm[q]];
+    g.ldb[q2] = fldb[q];
+    ++q2;
+  }
   return TTC_SUCCESS;
 }
 
 // ------------------------------------------------------------ vector widths
 static bool vec_ok_a(const Geometry& g, int w) {
+  if (w > 1 && g.sa[0] != 1) return false;
   if (g.size[0] % w) return false;
   for (int k = 1; k < g.rank; ++k)
     if (g.sa[k] % w) return false;
@@ -151,6 +169,7 @@ static bool vec_ok_a(const Geometry& g, int w) {
 }
 static bool vec_ok_b(const Geometry& g, int w) {
   const int p0 = g.perm[0];
+  if (w > 1 && g.sb[p0] != 1) return false;
   if (g.size[p0] % w) return false;
   for (int k = 0; k < g.rank; ++k)
     if (k != p0 && g.sb[k] % w) return false;
@@ -158,21 +177,23 @@ static bool vec_ok_b(const Geometry& g, int w) {
 }
 int widest_vec_a(const Geometry& g, int ta) {
   int best = 1;
+  if (g.elements == 0) return 1;
   for (int w : vec_widths(dtype_bytes(ta)))
     if (vec_ok_a(g, w)) best = std::max(best, w);
   return best;
 }
 int widest_vec_b(const Geometry& g, int tb) {
   int best = 1;
+  if (g.elements == 0) return 1;
   for (int w : vec_widths(dtype_bytes(tb)))
     if (vec_ok_b(g, w)) best = std::max(best, w);
   return best;
 }
 
-int max_tile_volume(int va, int vb, int lds_elem) {
+int max_tile_volume(int ta, int tb, int va, int vb) {
   const int kLdsBudget = 32768;  // bytes of LDS per work-group (5 work-groups per 160 KiB CU)
-  int v = kThreads * std::min(slots_for(va) * va, slots_for(vb) * vb);
-  return std::min(v, kLdsBudget / lds_elem);
+  const int v = kThreads * std::min(slots_for(va, dtype_bytes(ta)) * va, slots_for(vb, dtype_bytes(tb)) * vb);
+  return std::min(v, kLdsBudget / lds_elem_bytes(ta, tb));
 }
 
 // -------------------------------------------------------------- tile choice
@@ -290,7 +311,7 @@ double tile_score(const Geometry& g, const int* tile, int szA, int szB) {
 
 void choose_tile(const Geometry& g, int ta, int tb, int va, int vb, int run_a, int run_b, int* tile) {
   const int szA = dtype_bytes(ta), szB = dtype_bytes(tb);
-  const int vmax = max_tile_volume(va, vb, lds_elem_bytes(ta, tb));
+  const int vmax = max_tile_volume(ta, tb, va, vb);
   if (run_a > 0 && run_b > 0) {
     tile_for_runs(g, va, vb, vmax, run_a, run_b, tile);
     return;
@@ -312,8 +333,9 @@ void choose_tile(const Geometry& g, int ta, int tb, int va, int vb, int run_a, i
 }
 
 // ------------------------------------------------------------ launch params
-ttc_status_t build_params(const Geometry& g, const int* tile, int va, int vb, int lds_elem, int grid_order,
+ttc_status_t build_params(const Geometry& g, const int* tile, int ta, int tb, int va, int vb, int grid_order,
                           TileParams& p, int& lds_bytes, int& lds_pad) {
+  const int lds_elem = lds_elem_bytes(ta, tb);
   std::memset(&p, 0, sizeof(p));
   const int r = g.rank;
   p.rank = r;
@@ -348,8 +370,9 @@ ttc_status_t build_params(const Geometry& g, const int* tile, int va, int vb, in
       ++nsplit;
     }
   }
-  if (spanA >= INT_MAX || spanB >= INT_MAX) return TTC_ERROR_UNSUPPORTED;
-  if (V > max_tile_volume(va, vb, lds_elem)) return TTC_ERROR_UNSUPPORTED;
+  // tile-relative byte offsets are unsigned 32-bit in the kernels
+  if (spanA * dtype_bytes(ta) >= INT_MAX || spanB * dtype_bytes(tb) >= INT_MAX) return TTC_ERROR_UNSUPPORTED;
+  if (V > max_tile_volume(ta, tb, va, vb)) return TTC_ERROR_UNSUPPORTED;
   const int p0 = g.perm[0];
   if (tile[0] % va || tile[p0] % vb) return TTC_ERROR_INVALID_VALUE;
   if (g.size[0] % va || g.size[p0] % vb) return TTC_ERROR_INVALID_VALUE;
@@ -370,6 +393,33 @@ ttc_status_t build_params(const Geometry& g, const int* tile, int va, int vb, in
   for (int k = 0; k < r; ++k) elems += (long long)(tile[k] - 1) * p.ls[k];
   p.lds_elems = (int32_t)elems;
   p.lds_stride_b = p.ls[p0];
+  p.splitB0 = p.splitB1 = -1;
+  for (int q = 0; q < r; ++q) {
+    const int k = g.perm[q];
+    p.tdivB[q] = p.tdiv[k];
+    p.tB[q] = p.t[k];
+    p.sbB[q] = p.sb[k];
+    p.lsB[q] = p.ls[k];
+    if (k == p.split0) p.splitB0 = q;
+    if (k == p.split1) p.splitB1 = q;
+  }
+  // prefix products for the digits of the split dimensions
+  auto prefix_a = [&](int k) {
+    uint32_t P = 1;
+    for (int m = 0; m < k; ++m) P *= (uint32_t)tile[m];
+    return P;
+  };
+  auto prefix_b = [&](int k) {
+    uint32_t P = 1;
+    for (int q = 0; q < r && g.perm[q] != k; ++q) P *= (uint32_t)tile[g.perm[q]];
+    return P;
+  };
+  p.pA0 = make_fastdiv(p.split0 >= 0 ? prefix_a(p.split0) : 1);
+  p.pA1 = make_fastdiv(p.split1 >= 0 ? prefix_a(p.split1) : 1);
+  p.pB0 = make_fastdiv(p.split0 >= 0 ? prefix_b(p.split0) : 1);
+  p.pB1 = make_fastdiv(p.split1 >= 0 ? prefix_b(p.split1) : 1);
+  p.ts0 = make_fastdiv(p.split0 >= 0 ? (uint32_t)tile[p.split0] : 1);
+  p.ts1 = make_fastdiv(p.split1 >= 0 ? (uint32_t)tile[p.split1] : 1);
   lds_bytes = (int)((elems * lds_elem + 15) / 16 * 16);
   if (lds_bytes > 65536) return TTC_ERROR_UNSUPPORTED;
   p.grid_order = grid_order == 2 ? 1 : 0;
@@ -417,7 +467,7 @@ static ttc_status_t plan_fill(ttc_plan_s* pl, int va, int vb, const int* tile_in
   pl->va = va;
   pl->vb = vb;
   for (int k = 0; k < kMaxDim; ++k) pl->tile[k] = k < g.rank ? tile[k] : 1;
-  ttc_status_t st = build_params(g, tile, va, vb, lds_elem_bytes(pl->type_a, pl->type_b), pl->grid_order, pl->p,
+  ttc_status_t st = build_params(g, tile, pl->type_a, pl->type_b, va, vb, pl->grid_order, pl->p,
                                  pl->lds_bytes, pl->lds_pad);
   if (st != TTC_SUCCESS) return st;
   long long ra, rb;
@@ -484,7 +534,7 @@ extern "C" ttc_status_t ttc_plan_create(ttc_plan_t* plan, ttc_dtype_t type_a, tt
     const int* t = pl->tile;
     long long v = 1;
     for (int k = 0; k < pl->g.rank; ++k) v *= t[k];
-    const bool reuse = v <= max_tile_volume(1, 1, lds_elem_bytes(type_a, type_b));
+    const bool reuse = v <= max_tile_volume(type_a, type_b, 1, 1);
     st = plan_fill(fb, 1, 1, reuse ? t : nullptr, c.run_a, c.run_b);
     if (st != TTC_SUCCESS) {
       delete fb;

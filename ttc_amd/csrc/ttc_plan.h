@@ -52,7 +52,7 @@ struct PlanChoice {
 
 // Fills TileParams for a geometry and a tile; returns TTC_ERROR_UNSUPPORTED
 // when the tile violates a kernel limit.
-ttc_status_t build_params(const Geometry& g, const int* tile, int va, int vb, int lds_elem_bytes, int grid_order,
+ttc_status_t build_params(const Geometry& g, const int* tile, int ta, int tb, int va, int vb, int grid_order,
                           TileParams& p, int& lds_bytes, int& lds_pad);
 
 // Contiguous element runs of a full tile in A and in B.
@@ -65,8 +65,8 @@ void choose_tile(const Geometry& g, int ta, int tb, int va, int vb, int run_a, i
 // pointers is checked at execution).
 int widest_vec_a(const Geometry& g, int ta);
 int widest_vec_b(const Geometry& g, int tb);
-// Largest tile volume the kernel takes for these vector widths / LDS type.
-int max_tile_volume(int va, int vb, int lds_elem_bytes);
+// Largest tile volume the kernel takes for these vector widths and types.
+int max_tile_volume(int ta, int tb, int va, int vb);
 
 }  // namespace ttc
 

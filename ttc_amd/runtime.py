@@ -224,12 +224,36 @@ class Plan:
         _check(self._lib.ttc_plan_get_info(self._h, ctypes.byref(i)), "ttc_plan_get_info")
         return i.as_dict()
 
+    def extent_bytes(self):
+        """Bytes spanned by A and by B (last element offset + 1), from the
+        plan's strides; the C ABI takes raw pointers and cannot check."""
+        if getattr(self, "_extent", None) is None:
+            i = self.info()
+            if i["elements"] == 0:
+                self._extent = (0, 0)
+            else:
+                ea = sum((n - 1) * s for n, s in zip(i["size"], i["stride_a"])) + 1
+                eb = sum((n - 1) * s for n, s in zip(i["size"], i["stride_b"])) + 1
+                self._extent = (ea * DTYPE_BYTES[self.type_a], eb * DTYPE_BYTES[self.type_b])
+        return self._extent
+
+    def _check_buffers(self, A, B):
+        ea, eb = self.extent_bytes()
+        for name, t, need in (("A", A, ea), ("B", B, eb)):
+            if t is not None and not isinstance(t, int):
+                have = t.numel() * t.element_size() - (0 if t.is_contiguous() else need)
+                if not t.is_contiguous() or have < need:
+                    raise ValueError("buffer %s holds %d bytes (contiguous: %s) but the plan spans %d bytes"
+                                     % (name, t.numel() * t.element_size(), t.is_contiguous(), need))
+
     def execute(self, A, B, alpha=1.0, beta=0.0, stream=None):
+        self._check_buffers(A, B)
         _check(self._lib.ttc_execute(self._h, _ptr(A), _ptr(B), float(alpha), float(beta), _stream(stream)),
                "ttc_execute")
 
     def execute_fp8(self, A, B, alpha=1.0, scale=1.0, scale_in=None, amax_out=None, scale_inv_out=None,
                     mode=SCALE_DELAYED, stream=None):
+        self._check_buffers(A, B)
         _check(self._lib.ttc_execute_fp8(self._h, _ptr(A), _ptr(B), float(alpha), _ptr(scale_in), float(scale),
                                          _ptr(amax_out), _ptr(scale_inv_out), int(mode), _stream(stream)),
                "ttc_execute_fp8")
