@@ -217,8 +217,20 @@ static int align_of(const Geometry& g, int k, int va, int vb) {
   return a;
 }
 
+// Largest tile-relative byte offset on each side must fit the kernels'
+// unsigned 32-bit offsets (kept below 2^31).
+static bool span_ok(const Geometry& g, const int* tile, int szA, int szB) {
+  long long sa = 0, sb = 0;
+  for (int k = 0; k < g.rank; ++k) {
+    sa += (long long)(tile[k] - 1) * g.sa[k];
+    sb += (long long)(tile[k] - 1) * g.sb[k];
+  }
+  return sa * szA < INT_MAX && sb * szB < INT_MAX;
+}
+
 // Tile for given run targets (elements) on A's and B's side.
-static void tile_for_runs(const Geometry& g, int va, int vb, int vmax, int run_a, int run_b, int* tile) {
+static void tile_for_runs(const Geometry& g, int va, int vb, int vmax, int szA, int szB, int run_a, int run_b,
+                          int* tile) {
   const int r = g.rank;
   for (int k = 0; k < kMaxDim; ++k) tile[k] = 1;
   // A side: cover A's leading dimensions while they stay one contiguous run.
@@ -259,6 +271,21 @@ static void tile_for_runs(const Geometry& g, int va, int vb, int vmax, int run_a
     for (int k = 0; k < r; ++k) {
       const int a = align_of(g, k, va, vb);
       if (tile[k] > a && (best < 0 || tile[k] >= tile[best])) best = k;
+    }
+    if (best < 0) break;
+    const int a = align_of(g, best, va, vb);
+    tile[best] = std::max(a, round_up((tile[best] + 1) / 2, a));
+  }
+  // Shrink to the offset limit: halve the extent with the largest span.
+  while (!span_ok(g, tile, szA, szB)) {
+    int best = -1;
+    long long bspan = 0;
+    for (int k = 0; k < r; ++k) {
+      const long long sp = (long long)(tile[k] - 1) * std::max(g.sa[k] * szA, g.sb[k] * szB);
+      if (tile[k] > align_of(g, k, va, vb) && sp > bspan) {
+        bspan = sp;
+        best = k;
+      }
     }
     if (best < 0) break;
     const int a = align_of(g, best, va, vb);
@@ -313,7 +340,7 @@ void choose_tile(const Geometry& g, int ta, int tb, int va, int vb, int run_a, i
   const int szA = dtype_bytes(ta), szB = dtype_bytes(tb);
   const int vmax = max_tile_volume(ta, tb, va, vb);
   if (run_a > 0 && run_b > 0) {
-    tile_for_runs(g, va, vb, vmax, run_a, run_b, tile);
+    tile_for_runs(g, va, vb, vmax, szA, szB, run_a, run_b, tile);
     return;
   }
   // Search the run targets (powers of two) and keep the best-scoring tile.
@@ -322,7 +349,7 @@ void choose_tile(const Geometry& g, int ta, int tb, int va, int vb, int run_a, i
   for (int ra = 1; ra <= vmax; ra *= 2) {
     for (int rb = 1; rb <= vmax; rb *= 2) {
       if ((long long)ra * rb > 4LL * vmax) continue;
-      tile_for_runs(g, va, vb, vmax, run_a > 0 ? run_a : ra, run_b > 0 ? run_b : rb, cand);
+      tile_for_runs(g, va, vb, vmax, szA, szB, run_a > 0 ? run_a : ra, run_b > 0 ? run_b : rb, cand);
       const double sc = tile_score(g, cand, szA, szB);
       if (sc > best * (1.0 + 1e-9)) {
         best = sc;
