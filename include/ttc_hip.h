@@ -77,6 +77,7 @@ typedef struct {
   int grid_order;        /* 0 = auto (strided), 1 = strided tile walk, 2 = contiguous tile ranges */
   int run_a;             /* heuristic target run length on A's side, elements (0 = default) */
   int run_b;             /* heuristic target run length on B's side, elements (0 = default) */
+  int family;            /* kernel family: 0 = auto, 1 = box tile, 2 = row x column tile */
 } ttc_config_t;
 
 /* What a plan resolved to (fused geometry, tile, launch shape). */
@@ -102,6 +103,8 @@ typedef struct {
   int type_a, type_b;
   int beta_is_zero;
   int run_a, run_b;              /* contiguous element runs of a full tile in A and B */
+  int family;                    /* 1 = box tile kernel, 2 = row x column tile kernel */
+  int rows, cols;                /* row x column tile: row and column counts (0 for box tiles) */
 } ttc_plan_info_t;
 
 /* Library version string. */

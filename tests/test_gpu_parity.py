@@ -70,12 +70,30 @@ RANDOM_SHAPES = [
 ]
 
 
+def families(tA, tB, perm, size, lda=None, ldb=None):
+    """Kernel families a shape can run on: the box tile always, the row x
+    column tile when A's and B's leading dimensions differ."""
+    from ttc_amd import runtime as rt
+    try:
+        rt.Plan(tA, tB, perm, size, lda, ldb, True, dict(family=2))
+        return [1, 2]
+    except rt.TTCError as e:
+        assert e.status == rt.ERROR_UNSUPPORTED
+        return [1]
+
+
 @pytest.mark.parametrize("code", ["s", "d", "c", "z", "sd", "ds", "cz", "zc", "b", "h", "sb", "sh", "bs", "hs"])
 def test_random_shapes(gpu, code):
     tA, tB = CODE[code]
+    n_rc = 0
     for i, (perm, size) in enumerate(RANDOM_SHAPES):
-        for beta in (0.0, 0.7):
-            check(gpu, tA, tB, perm, size, 1.3, beta, seed=100 + i, dist="outliers")
+        for fam in families(tA, tB, perm, size):
+            n_rc += fam == 2
+            for beta in (0.0, 0.7):
+                info = check(gpu, tA, tB, perm, size, 1.3, beta, seed=100 + i, dist="outliers",
+                             config=dict(family=fam))
+                assert info["family"] == fam
+    assert n_rc >= 9
 
 
 def test_padded_subtensor(gpu):
@@ -90,9 +108,11 @@ def test_padded_subtensor(gpu):
         assert max(na, nb) < 40_000_000, (size, lda, ldb)
         for code in ("s", "d", "z", "ds"):
             tA, tB = CODE[code]
-            check(gpu, tA, tB, perm, size, 1.0, 1.0, lda, ldb, seed=i)
-            check(gpu, tA, tB, perm, size, 1.1, 0.0, lda, None, seed=i)
-            check(gpu, tA, tB, perm, size, 0.9, -1.0, None, ldb, seed=i)
+            for fam in families(tA, tB, perm, size, lda, ldb):
+                cfg = dict(family=fam)
+                check(gpu, tA, tB, perm, size, 1.0, 1.0, lda, ldb, seed=i, config=cfg)
+                check(gpu, tA, tB, perm, size, 1.1, 0.0, lda, None, seed=i, config=cfg)
+                check(gpu, tA, tB, perm, size, 0.9, -1.0, None, ldb, seed=i, config=cfg)
 
 
 def test_every_vector_width_and_mode(gpu):
@@ -105,9 +125,12 @@ def test_every_vector_width_and_mode(gpu):
             for vb in vec_widths(rt.DTYPE_BYTES[tB]):
                 for nt in (1, 2):
                     for order in (1, 2):
-                        cfg = dict(vec_a=va, vec_b=vb, nt_store=nt, grid_order=order, wg_per_cu=1)
-                        for beta in (0.0, 1.5):
-                            check(gpu, tA, tB, perm, size, 1.25, beta, config=cfg, seed=va * 10 + vb)
+                        for fam in (1, 2):
+                            cfg = dict(vec_a=va, vec_b=vb, nt_store=nt, grid_order=order, wg_per_cu=1,
+                                       family=fam)
+                            for beta in (0.0, 1.5):
+                                info = check(gpu, tA, tB, perm, size, 1.25, beta, config=cfg, seed=va * 10 + vb)
+                                assert info["family"] == fam and info["vec_a"] == va and info["vec_b"] == vb
 
 
 def test_explicit_tiles_with_two_partial_dims(gpu):
@@ -117,11 +140,39 @@ def test_explicit_tiles_with_two_partial_dims(gpu):
                                                               vec_b=1 if tile[1] % 2 else 2))
 
 
+def test_explicit_row_column_tiles(gpu):
+    """Row x column tiles with partial last tiles on the row group, the
+    column group or both, multi-dimension groups and non-dense tables."""
+    cases = [([1, 0, 2], [100, 90, 7], [36, 28, 1]), ([1, 0, 2], [100, 90, 7], [50, 7, 1]),
+             ([3, 1, 0, 2], [12, 10, 9, 11], [12, 4, 1, 5]), ([3, 1, 0, 2], [12, 10, 9, 11], [12, 10, 3, 11]),
+             ([3, 2, 1, 0], [9, 10, 11, 12], [9, 4, 1, 7]), ([1, 0], [1531, 977], [96, 80]),
+             ([1, 0], [1531, 977], [1531, 4])]
+    from ttc_amd import runtime as rt
+    n = 0
+    for perm, size, tile in cases:
+        for code in ("s", "d", "z", "bs"):
+            tA, tB = CODE[code]
+            cfg = dict(tile=tile, family=2, vec_a=1, vec_b=1)
+            # volume limit of the family: 256 threads x (128 bytes, at most 32 elements) each side
+            limit = 256 * min(min(32, 128 // rt.DTYPE_BYTES[t]) for t in (tA, tB))
+            if int(np.prod(tile)) > limit:
+                with pytest.raises(rt.TTCError):
+                    rt.Plan(tA, tB, perm, size, None, None, True, cfg)
+                continue
+            for beta in (0.0, 1.0):
+                info = check(gpu, tA, tB, perm, size, 1.0, beta, config=cfg)
+                assert info["family"] == 2 and info["tile"] == tile
+                n += 1
+    assert n >= 40
+
+
 def test_misaligned_pointers_use_scalar_variant(gpu):
     for off in (1, 3):
-        check(gpu, S, S, [1, 0], [256, 192], 1.0, 1.0, offset_a=off, offset_b=0)
-        check(gpu, S, S, [1, 0], [256, 192], 1.0, 0.0, offset_a=0, offset_b=off)
-        check(gpu, BF16, BF16, [1, 0], [256, 192], 1.0, 0.0, offset_a=off, offset_b=off)
+        for fam in (1, 2):
+            cfg = dict(family=fam)
+            check(gpu, S, S, [1, 0], [256, 192], 1.0, 1.0, offset_a=off, offset_b=0, config=cfg)
+            check(gpu, S, S, [1, 0], [256, 192], 1.0, 0.0, offset_a=0, offset_b=off, config=cfg)
+            check(gpu, BF16, BF16, [1, 0], [256, 192], 1.0, 0.0, offset_a=off, offset_b=off, config=cfg)
 
 
 def test_edge_shapes(gpu):

@@ -111,12 +111,46 @@ def _check_plan(i, perm, size):
     assert i["size"][0] % i["vec_a"] == 0 and i["size"][i["perm"][0]] % i["vec_b"] == 0
     def slots(vec, nbytes):  # ttc_params.h slots_for
         return min(16, max(4, 64 // (vec * nbytes)))
-    assert i["slots_a"] <= slots(i["vec_a"], rt.DTYPE_BYTES[i["type_a"]])
-    assert i["slots_b"] <= slots(i["vec_b"], rt.DTYPE_BYTES[i["type_b"]])
+    def rc_elems(nbytes):  # ttc_params.h rc_elems
+        return min(32, 128 // nbytes)
+    if i["family"] == 1:
+        assert i["slots_a"] <= slots(i["vec_a"], rt.DTYPE_BYTES[i["type_a"]])
+        assert i["slots_b"] <= slots(i["vec_b"], rt.DTYPE_BYTES[i["type_b"]])
+        assert i["rows"] == 0 and i["cols"] == 0
+    else:
+        assert i["family"] == 2
+        ea, eb = rc_elems(rt.DTYPE_BYTES[i["type_a"]]), rc_elems(rt.DTYPE_BYTES[i["type_b"]])
+        assert i["slots_a"] * i["vec_a"] == ea and i["slots_b"] * i["vec_b"] == eb
+        assert V <= 256 * min(ea, eb)
+        _check_rc_groups(i)
     assert 0 < i["lds_bytes"] <= 65536
     partial = sum(1 for k in range(r) if i["size"][k] % i["tile"][k])
     assert partial <= 2
     assert i["elements"] == int(np.prod(size))
+
+
+def _check_rc_groups(i):
+    """A row x column tile: some split into rows = A's leading dimensions
+    0..na-1 (never B's stride-1 dimension) and columns = B's leading
+    dimensions perm[0..nb-1], disjoint, covering every dimension with t > 1,
+    with a partial last tile only on a group's outermost dimension, whose
+    row and column counts are the plan's."""
+    r, t, n, perm = i["rank"], i["tile"], i["size"], i["perm"]
+    assert perm[0] != 0
+    big = {k for k in range(r) if t[k] > 1}
+    found = False
+    for na in range(1, r + 1):
+        ga = list(range(na))
+        for nb in range(1, r + 1):
+            gb = list(perm[:nb])
+            if set(ga) & set(gb) or not big <= set(ga) | set(gb):
+                continue
+            if any(n[k] % t[k] for k in ga[:-1] + gb[:-1]):
+                continue
+            rows, cols = int(np.prod([t[k] for k in ga])), int(np.prod([t[k] for k in gb]))
+            if rows == i["rows"] and cols == i["cols"] and rows * cols == i["tile_volume"]:
+                found = True
+    assert found, (t, n, perm, i["rows"], i["cols"])
 
 
 @pytest.mark.parametrize("code", ["s", "d", "c", "z", "sd", "ds", "cz", "zc", "b", "h", "sb", "sh", "bs", "hs"])

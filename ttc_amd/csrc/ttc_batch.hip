@@ -280,8 +280,10 @@ extern "C" ttc_status_t ttc_batch_create(ttc_batch_t* out, int nops, const ttc_b
       any_lda |= op.lda[k] != 0;
       any_ldb |= op.ldb[k] != 0;
     }
-    // vector widths: 1 or the widest of the type (the kinds instantiated)
+    // vector widths: 1 or the widest of the type (the kinds instantiated);
+    // the grouped kernels run box tiles
     ttc_config_t cfg = op.cfg;
+    cfg.family = kFamilyBox;
     const int wide = std::max(1, 16 / dtype_bytes(op.type_a));
     for (int pass = 0; pass < 2; ++pass) {
       ttc_plan_t pl = nullptr;

@@ -5,20 +5,27 @@
 #include <hip/hip_runtime.h>
 
 #include "ttc_kernels.hpp"
+#include "ttc_rc.hpp"
 
 namespace ttc {
 
 // mode of a transpose entry
 enum : int { kModeBeta = 0, kModeBz = 1, kModeBzNt = 2, kModeFp8 = 3 };
+// kernel family (matches ttc_config_t.family)
+enum : int { kFamBox = 1, kFamRC = 2 };
 
 using LaunchFn = hipError_t (*)(const TileParams&, const void*, void*, double, double, unsigned, size_t, hipStream_t);
 using LaunchFp8Fn = hipError_t (*)(const TileParams&, const void*, void*, float, const float*, const float*, float,
                                    float, float*, float*, unsigned, size_t, hipStream_t);
 
+using LaunchRCFn = hipError_t (*)(const RCParams&, const void*, void*, double, double, unsigned, size_t,
+                                  hipStream_t);
+
 struct KernelEntry {
-  int ta, tb, va, vb, mode;
+  int ta, tb, va, vb, mode, family;
   LaunchFn launch;
   LaunchFp8Fn launch_fp8;
+  LaunchRCFn launch_rc;
   const void* func;
 };
 
@@ -45,9 +52,30 @@ hipError_t launch_fp8(const TileParams& p, const void* A, void* B, float alpha, 
                          args, lds, s);
 }
 
+template <class OP, int VA, int VB, bool BETA, bool NT>
+hipError_t launch_rc(const RCParams& p, const void* A, void* B, double alpha, double beta, unsigned grid, size_t lds,
+                     hipStream_t s) {
+  const typename OP::TA* a = static_cast<const typename OP::TA*>(A);
+  typename OP::TB* b = static_cast<typename OP::TB*>(B);
+  RCParams pp = p;
+  void* args[] = {&pp, &a, &b, &alpha, &beta};
+  return hipLaunchKernel(reinterpret_cast<const void*>(&transpose_rc_kernel<OP, VA, VB, BETA, NT>), dim3(grid),
+                         dim3(kThreads), args, lds, s);
+}
+
+template <class OP, int VA, int VB, int MODE>
+KernelEntry make_rc_entry(int ta, int tb) {
+  constexpr bool beta = MODE == kModeBeta;
+  constexpr bool nt = MODE == kModeBzNt;
+  KernelEntry e{ta, tb, VA, VB, MODE, kFamRC, nullptr, nullptr, nullptr, nullptr};
+  e.launch_rc = &launch_rc<OP, VA, VB, beta, nt>;
+  e.func = reinterpret_cast<const void*>(&transpose_rc_kernel<OP, VA, VB, beta, nt>);
+  return e;
+}
+
 template <class OP, int VA, int VB, int MODE>
 KernelEntry make_entry(int ta, int tb) {
-  KernelEntry e{ta, tb, VA, VB, MODE, nullptr, nullptr, nullptr};
+  KernelEntry e{ta, tb, VA, VB, MODE, kFamBox, nullptr, nullptr, nullptr, nullptr};
   if constexpr (MODE == kModeFp8) {
     e.launch_fp8 = &launch_fp8<OP, VA, VB>;
     e.func = reinterpret_cast<const void*>(&cast_fp8_kernel<OP, VA, VB>);
@@ -61,6 +89,6 @@ KernelEntry make_entry(int ta, int tb) {
 }
 
 // Registry access (ttc_runtime.hip).
-const KernelEntry* find_kernel(int ta, int tb, int va, int vb, int mode);
+const KernelEntry* find_kernel(int ta, int tb, int va, int vb, int mode, int family);
 
 }  // namespace ttc

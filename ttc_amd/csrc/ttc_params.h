@@ -73,4 +73,37 @@ struct TileParams {
   int32_t grid_order;          // 0 strided walk, 1 contiguous ranges
 };
 
+// Row x column tile (ttc_rc.hpp): the tile's dimensions split into an A
+// group (A's leading dimensions, the rows) and a disjoint B group (B's
+// leading dimensions, the columns); only the outermost dimension of each
+// group may have a partial last tile.
+struct RCParams {
+  int32_t R, C;          // rows (A-group volume) and columns (B-group volume) of a full tile
+  int32_t Rl, Cl;        // rows / columns of the last tile along the split dims (== R / C if none)
+  FastDiv dR, dRl, dC, dCl;
+  int32_t RP;            // LDS pitch of a column (>= R)
+  int32_t rowA_dense;    // rowA[i] == i (read vectors need this)
+  int32_t colB_dense;    // colB[j] == j (write vectors need this)
+  int32_t na, nb;        // dimensions in the A group / B group
+  int32_t ta[kMaxDim];   // tile extents of the A group, A order
+  FastDiv dta[kMaxDim];
+  int32_t saA[kMaxDim], sbA[kMaxDim];  // A and B strides of the A-group dims
+  int32_t tb[kMaxDim];   // tile extents of the B group, B order
+  FastDiv dtb[kMaxDim];
+  int32_t saB[kMaxDim], sbB[kMaxDim];  // A and B strides of the B-group dims
+  // tile grid over all dimensions (A order)
+  int32_t rank;
+  int32_t n[kMaxDim], t[kMaxDim];
+  FastDiv gdiv[kMaxDim];
+  int64_t tsa[kMaxDim], tsb[kMaxDim];
+  int32_t splitA, splitB;  // the partial dim of the A group / B group (-1 none)
+  int32_t tsA, tsB;        // their tile extents
+  uint32_t ntiles;
+  int32_t grid_order;
+  int32_t lds_data_elems;  // RP * C
+};
+
+// Elements per thread per tile side: as many as fit 128 bytes, at most 32.
+constexpr int rc_elems(int bytes) { return 128 / bytes > 32 ? 32 : 128 / bytes; }
+
 }  // namespace ttc

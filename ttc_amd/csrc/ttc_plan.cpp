@@ -130,10 +130,14 @@ ttc_status_t make_geometry(int dim, const int* perm, const int* size, const int*
   }
   for (int q = 0; q < r; ++q) sb[fperm[q]] = sbpos[q];
   // Extent-1 dimensions index nothing: drop them (their padding is already
-  // in the strides of the others), except A's and B's stride-1 dimensions.
+  // in the strides of the others); a tensor of one element keeps dimension 0.
+  // The new leading dimensions may then have strides other than 1, which the
+  // vector-width checks see.
   int keep[TTC_MAX_INPUT_DIM], newid[TTC_MAX_INPUT_DIM], nk = 0;
+  bool all_one = true;
+  for (int k = 0; k < r; ++k) all_one &= fsize[k] == 1;
   for (int k = 0; k < r; ++k) {
-    keep[k] = fsize[k] != 1 || k == 0 || k == fperm[0];
+    keep[k] = fsize[k] != 1 || (all_one && k == 0);
     newid[k] = keep[k] ? nk++ : -1;
   }
   if (nk > kMaxDim) return TTC_ERROR_UNSUPPORTED;
@@ -453,6 +457,151 @@ ttc_status_t build_params(const Geometry& g, const int* tile, int ta, int tb, in
   return TTC_SUCCESS;
 }
 
+// ------------------------------------------------------ row x column family
+// LDS per work-group (tile data plus offset tables): four work-groups per CU.
+static constexpr int kRcLdsBudget = 40960;
+
+int max_rc_volume(int ta, int tb) {
+  const int v = kThreads * std::min(rc_elems(dtype_bytes(ta)), rc_elems(dtype_bytes(tb)));
+  return std::min(v, 32768 / lds_elem_bytes(ta, tb));
+}
+
+static long long rc_lds(long long R, long long C, int lds_elem) {
+  const long long RP = R | 1;  // odd pitch: the write phase reads LDS with stride RP
+  return (RP * C * lds_elem + 15) / 16 * 16 + (2 * R + 2 * C) * 4;
+}
+
+// Rows: A's leading dimensions 0, 1, ... up to B's stride-1 dimension;
+// columns: B's leading dimensions perm[0], perm[1], ... up to a row
+// dimension.  A group stops after a dimension whose extent the tile does not
+// divide (only a group's outermost dimension may have a partial last tile)
+// and before a dimension the tile does not cover (t == 1).  Every dimension
+// with t > 1 must land in a group.
+bool rc_groups(const Geometry& g, const int* tile, int& na, int* ga, int& nb, int* gb) {
+  const int r = g.rank;
+  const int p0 = g.perm[0];
+  na = nb = 0;
+  if (p0 == 0 || r < 2) return false;
+  bool in_a[kMaxDim] = {false}, in_b[kMaxDim] = {false};
+  for (int k = 0; k < r && k != p0; ++k) {
+    if (k > 0 && (tile[k] == 1 || g.size[k - 1] % tile[k - 1] != 0)) break;
+    ga[na++] = k;
+    in_a[k] = true;
+  }
+  for (int q = 0; q < r; ++q) {
+    const int k = g.perm[q];
+    if (in_a[k]) break;
+    if (q > 0) {
+      const int kp = g.perm[q - 1];
+      if (tile[k] == 1 || g.size[kp] % tile[kp] != 0) break;
+    }
+    gb[nb++] = k;
+    in_b[k] = true;
+  }
+  for (int k = 0; k < r; ++k)
+    if (tile[k] > 1 && !in_a[k] && !in_b[k]) return false;
+  return true;
+}
+
+ttc_status_t build_rc_params(const Geometry& g, const int* tile, int ta, int tb, int va, int vb, int grid_order,
+                             RCParams& p, int& lds_bytes) {
+  int na, nb, ga[kMaxDim], gb[kMaxDim];
+  if (!rc_groups(g, tile, na, ga, nb, gb)) return TTC_ERROR_INVALID_VALUE;
+  std::memset(&p, 0, sizeof(p));
+  const int r = g.rank;
+  long long ntiles = 1, spanA = 0, spanB = 0;
+  for (int k = 0; k < r; ++k) {
+    const int n = g.size[k], t = tile[k];
+    if (t < 1 || (n > 0 && t > n)) return TTC_ERROR_INVALID_VALUE;
+    spanA += (long long)(t - 1) * g.sa[k];
+    spanB += (long long)(t - 1) * g.sb[k];
+    p.n[k] = n;
+    p.t[k] = t;
+    p.tsa[k] = (int64_t)t * g.sa[k];
+    p.tsb[k] = (int64_t)t * g.sb[k];
+    const long long gk = n > 0 ? (n + t - 1) / t : 1;
+    if (n == 0) ntiles = 0;
+    ntiles *= gk;
+    if (ntiles > INT_MAX) return TTC_ERROR_UNSUPPORTED;
+    p.gdiv[k] = make_fastdiv((uint32_t)gk);
+  }
+  if (spanA * dtype_bytes(ta) >= INT_MAX || spanB * dtype_bytes(tb) >= INT_MAX) return TTC_ERROR_UNSUPPORTED;
+  const int p0 = g.perm[0];
+  if (tile[0] % va || tile[p0] % vb || g.size[0] % va || g.size[p0] % vb) return TTC_ERROR_INVALID_VALUE;
+  long long R = 1, C = 1;
+  p.rowA_dense = 1;
+  p.na = na;
+  for (int m = 0; m < na; ++m) {
+    const int k = ga[m];
+    if (g.sa[k] != R) p.rowA_dense = 0;
+    p.ta[m] = tile[k];
+    p.dta[m] = make_fastdiv((uint32_t)tile[k]);
+    p.saA[m] = (int32_t)g.sa[k];
+    p.sbA[m] = (int32_t)g.sb[k];
+    R *= tile[k];
+  }
+  p.colB_dense = 1;
+  p.nb = nb;
+  for (int m = 0; m < nb; ++m) {
+    const int k = gb[m];
+    if (g.sb[k] != C) p.colB_dense = 0;
+    p.tb[m] = tile[k];
+    p.dtb[m] = make_fastdiv((uint32_t)tile[k]);
+    p.saB[m] = (int32_t)g.sa[k];
+    p.sbB[m] = (int32_t)g.sb[k];
+    C *= tile[k];
+  }
+  if (R * C > max_rc_volume(ta, tb)) return TTC_ERROR_UNSUPPORTED;
+  const long long lds = rc_lds(R, C, lds_elem_bytes(ta, tb));
+  if (lds > kRcLdsBudget) return TTC_ERROR_UNSUPPORTED;
+  const int ka = ga[na - 1], kb = gb[nb - 1];
+  p.splitA = g.size[ka] > 0 && g.size[ka] % tile[ka] ? ka : -1;
+  p.splitB = g.size[kb] > 0 && g.size[kb] % tile[kb] ? kb : -1;
+  p.tsA = tile[ka];
+  p.tsB = tile[kb];
+  p.R = (int32_t)R;
+  p.C = (int32_t)C;
+  p.Rl = p.splitA >= 0 ? (int32_t)(R / tile[ka] * (g.size[ka] % tile[ka])) : p.R;
+  p.Cl = p.splitB >= 0 ? (int32_t)(C / tile[kb] * (g.size[kb] % tile[kb])) : p.C;
+  p.dR = make_fastdiv((uint32_t)p.R);
+  p.dRl = make_fastdiv((uint32_t)p.Rl);
+  p.dC = make_fastdiv((uint32_t)p.C);
+  p.dCl = make_fastdiv((uint32_t)p.Cl);
+  p.RP = (int32_t)(R | 1);
+  p.rank = r;
+  p.ntiles = (uint32_t)ntiles;
+  p.grid_order = grid_order == 2 ? 1 : 0;
+  p.lds_data_elems = (int32_t)(((long long)p.RP * C * lds_elem_bytes(ta, tb) + 15) / 16 * 16 /
+                               lds_elem_bytes(ta, tb));
+  lds_bytes = (int)lds;
+  return TTC_SUCCESS;
+}
+
+bool choose_rc_tile(const Geometry& g, int ta, int tb, int va, int vb, int run_a, int run_b, int* tile,
+                    double* score) {
+  if (g.elements == 0 || g.rank < 2 || g.perm[0] == 0) return false;
+  const int szA = dtype_bytes(ta), szB = dtype_bytes(tb);
+  const int vmax = max_rc_volume(ta, tb);
+  double best = -1.0;
+  int cand[kMaxDim];
+  RCParams p;
+  int lds;
+  for (int ra = 1; ra <= vmax; ra *= 2) {
+    for (int rb = 1; rb <= vmax; rb *= 2) {
+      if ((long long)ra * rb > 4LL * vmax) continue;
+      tile_for_runs(g, va, vb, vmax, szA, szB, run_a > 0 ? run_a : ra, run_b > 0 ? run_b : rb, cand);
+      if (build_rc_params(g, cand, ta, tb, va, vb, 0, p, lds) != TTC_SUCCESS) continue;
+      const double sc = tile_score(g, cand, szA, szB);
+      if (sc > best * (1.0 + 1e-9)) {
+        best = sc;
+        for (int k = 0; k < kMaxDim; ++k) tile[k] = cand[k];
+      }
+    }
+  }
+  if (score) *score = best;
+  return best >= 0.0;
+}
+
 }  // namespace ttc
 
 // ================================================================== C ABI
@@ -480,22 +629,47 @@ extern "C" ttc_status_t ttc_fuse_indices(int dim, const int* perm, const int* si
   return fuse_indices(dim, perm, size, lda, ldb, out_dim, out_perm, out_size, out_lda, out_ldb);
 }
 
-static ttc_status_t plan_fill(ttc_plan_s* pl, int va, int vb, const int* tile_in, int run_a, int run_b) {
+static ttc_status_t plan_fill(ttc_plan_s* pl, int va, int vb, const int* tile_in, int run_a, int run_b,
+                              int family) {
   const Geometry& g = pl->g;
   int tile[kMaxDim];
   bool explicit_tile = false;
   if (tile_in)
     for (int k = 0; k < g.rank; ++k) explicit_tile |= tile_in[k] != 0;
+  const bool fp8 = pl->type_b == TTC_E4M3 || pl->type_b == TTC_E5M2;
+  int fam = family;
   if (explicit_tile) {
     for (int k = 0; k < kMaxDim; ++k) tile[k] = k < g.rank ? tile_in[k] : 1;
+    if (fam == 0) fam = kFamilyBox;
   } else {
-    choose_tile(g, pl->type_a, pl->type_b, va, vb, run_a, run_b, tile);
+    int rc_tile[kMaxDim];
+    double rc_score = -1.0;
+    const bool rc_ok = !fp8 && fam != kFamilyBox &&
+                       choose_rc_tile(g, pl->type_a, pl->type_b, va, vb, run_a, run_b, rc_tile, &rc_score);
+    if (fam == kFamilyRC && !rc_ok) return TTC_ERROR_UNSUPPORTED;
+    if (fam == 0) fam = rc_ok ? kFamilyRC : kFamilyBox;
+    if (fam == kFamilyRC)
+      for (int k = 0; k < kMaxDim; ++k) tile[k] = rc_tile[k];
+    else
+      choose_tile(g, pl->type_a, pl->type_b, va, vb, run_a, run_b, tile);
   }
+  if (fam == kFamilyRC && fp8) return TTC_ERROR_UNSUPPORTED;
   pl->va = va;
   pl->vb = vb;
+  pl->family = fam;
   for (int k = 0; k < kMaxDim; ++k) pl->tile[k] = k < g.rank ? tile[k] : 1;
-  ttc_status_t st = build_params(g, tile, pl->type_a, pl->type_b, va, vb, pl->grid_order, pl->p,
-                                 pl->lds_bytes, pl->lds_pad);
+  ttc_status_t st;
+  if (fam == kFamilyRC) {
+    std::memset(&pl->p, 0, sizeof(pl->p));
+    st = build_rc_params(g, tile, pl->type_a, pl->type_b, va, vb, pl->grid_order, pl->rc, pl->lds_bytes);
+    pl->lds_pad = pl->rc.RP != pl->rc.R;
+    pl->p.ntiles = pl->rc.ntiles;
+    pl->p.V = pl->rc.R * pl->rc.C;
+    pl->p.grid_order = pl->rc.grid_order;
+  } else {
+    std::memset(&pl->rc, 0, sizeof(pl->rc));
+    st = build_params(g, tile, pl->type_a, pl->type_b, va, vb, pl->grid_order, pl->p, pl->lds_bytes, pl->lds_pad);
+  }
   if (st != TTC_SUCCESS) return st;
   long long ra, rb;
   tile_runs(g, tile, ra, rb);
@@ -516,7 +690,7 @@ extern "C" ttc_status_t ttc_plan_create(ttc_plan_t* plan, ttc_dtype_t type_a, tt
   std::memset(&c, 0, sizeof(c));
   if (cfg) c = *cfg;
   if (c.nt_store < 0 || c.nt_store > 2 || c.grid_order < 0 || c.grid_order > 2 || c.wg_per_cu < 0 || c.vec_a < 0 ||
-      c.vec_b < 0)
+      c.vec_b < 0 || c.family < 0 || c.family > 2)
     return TTC_ERROR_INVALID_VALUE;
   ttc_plan_s* pl = new (std::nothrow) ttc_plan_s();
   if (!pl) return TTC_ERROR_ALLOC;
@@ -545,7 +719,7 @@ extern "C" ttc_status_t ttc_plan_create(ttc_plan_t* plan, ttc_dtype_t type_a, tt
     delete pl;
     return TTC_ERROR_INVALID_VALUE;
   }
-  st = plan_fill(pl, va, vb, c.tile, c.run_a, c.run_b);
+  st = plan_fill(pl, va, vb, c.tile, c.run_a, c.run_b, c.family);
   if (st != TTC_SUCCESS) {
     delete pl;
     return st;
@@ -561,8 +735,8 @@ extern "C" ttc_status_t ttc_plan_create(ttc_plan_t* plan, ttc_dtype_t type_a, tt
     const int* t = pl->tile;
     long long v = 1;
     for (int k = 0; k < pl->g.rank; ++k) v *= t[k];
-    const bool reuse = v <= max_tile_volume(type_a, type_b, 1, 1);
-    st = plan_fill(fb, 1, 1, reuse ? t : nullptr, c.run_a, c.run_b);
+    const bool reuse = pl->family == kFamilyRC || v <= max_tile_volume(type_a, type_b, 1, 1);
+    st = plan_fill(fb, 1, 1, reuse ? t : nullptr, c.run_a, c.run_b, pl->family);
     if (st != TTC_SUCCESS) {
       delete fb;
       delete pl;
@@ -606,6 +780,13 @@ extern "C" ttc_status_t ttc_plan_get_info(ttc_plan_t pl, ttc_plan_info_t* info) 
   info->beta_is_zero = pl->beta_is_zero;
   info->run_a = pl->run_a;
   info->run_b = pl->run_b;
+  info->family = pl->family;
+  if (pl->family == kFamilyRC) {
+    info->slots_a = rc_elems(sa) / pl->va;
+    info->slots_b = rc_elems(sb) / pl->vb;
+    info->rows = pl->rc.R;
+    info->cols = pl->rc.C;
+  }
   return TTC_SUCCESS;
 }
 

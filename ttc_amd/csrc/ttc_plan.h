@@ -43,6 +43,10 @@ ttc_status_t fuse_indices(int dim, const int* perm, const int* size, const int* 
 // Validation (ttc/ttc.py:733-768) + fusion + strides.
 ttc_status_t make_geometry(int dim, const int* perm, const int* size, const int* lda, const int* ldb, Geometry& g);
 
+// Kernel families (ttc_config_t.family).
+constexpr int kFamilyBox = 1;
+constexpr int kFamilyRC = 2;
+
 struct PlanChoice {
   int tile[kMaxDim];
   int va, vb;
@@ -68,6 +72,16 @@ int widest_vec_b(const Geometry& g, int tb);
 // Largest tile volume the kernel takes for these vector widths and types.
 int max_tile_volume(int ta, int tb, int va, int vb);
 
+// Row x column family: largest tile volume, the group split of a tile (false
+// when the tile does not have the row x column shape), its launch parameters
+// and the heuristic tile (false when no row x column tile fits).
+int max_rc_volume(int ta, int tb);
+bool rc_groups(const Geometry& g, const int* tile, int& na, int* ga, int& nb, int* gb);
+ttc_status_t build_rc_params(const Geometry& g, const int* tile, int ta, int tb, int va, int vb, int grid_order,
+                             RCParams& p, int& lds_bytes);
+bool choose_rc_tile(const Geometry& g, int ta, int tb, int va, int vb, int run_a, int run_b, int* tile,
+                    double* score);
+
 }  // namespace ttc
 
 struct ttc_plan_s {
@@ -81,5 +95,7 @@ struct ttc_plan_s {
   ttc::TileParams p;
   int lds_bytes, lds_pad;
   int run_a, run_b;
+  int family;            // 1 box tile kernel (p), 2 row x column kernel (rc)
+  ttc::RCParams rc;
   ttc_plan_s* fallback;  // scalar-vector plan for misaligned pointers (may be null)
 };

@@ -32,9 +32,9 @@ static const std::vector<KernelEntry>& registry() {
   return all;
 }
 
-const KernelEntry* find_kernel(int ta, int tb, int va, int vb, int mode) {
+const KernelEntry* find_kernel(int ta, int tb, int va, int vb, int mode, int family) {
   for (const KernelEntry& e : registry())
-    if (e.ta == ta && e.tb == tb && e.va == va && e.vb == vb && e.mode == mode) return &e;
+    if (e.ta == ta && e.tb == tb && e.va == va && e.vb == vb && e.mode == mode && e.family == family) return &e;
   return nullptr;
 }
 
@@ -99,11 +99,13 @@ ttc_status_t launch_plan(const ttc_plan_s* pl, const void* A, void* B, double al
   const ttc_plan_s* use = variant_for(pl, A, B);
   if (!use) return TTC_ERROR_INVALID_VALUE;
   const int mode = pl->beta_is_zero ? (pl->nt_store ? kModeBzNt : kModeBz) : kModeBeta;
-  const KernelEntry* e = find_kernel(pl->type_a, pl->type_b, use->va, use->vb, mode);
+  const KernelEntry* e = find_kernel(pl->type_a, pl->type_b, use->va, use->vb, mode, use->family);
   if (!e) return TTC_ERROR_UNSUPPORTED;
   const unsigned grid = grid_size(use->p.ntiles, pl->wg_per_cu, e->func, use->lds_bytes);
-  return e->launch(use->p, A, B, alpha, beta, grid, (size_t)use->lds_bytes, s) == hipSuccess ? TTC_SUCCESS
-                                                                                            : TTC_ERROR_HIP;
+  const hipError_t err = use->family == kFamRC
+                             ? e->launch_rc(use->rc, A, B, alpha, beta, grid, (size_t)use->lds_bytes, s)
+                             : e->launch(use->p, A, B, alpha, beta, grid, (size_t)use->lds_bytes, s);
+  return err == hipSuccess ? TTC_SUCCESS : TTC_ERROR_HIP;
 }
 
 // ------------------------------------------------------------------- amax
@@ -177,7 +179,7 @@ extern "C" ttc_status_t ttc_execute_fp8(ttc_plan_t pl, const void* A, void* B, d
   if (!A || !B) return TTC_ERROR_INVALID_VALUE;
   const ttc_plan_s* use = variant_for(pl, A, B);
   if (!use) return TTC_ERROR_INVALID_VALUE;
-  const KernelEntry* e = find_kernel(pl->type_a, pl->type_b, use->va, use->vb, kModeFp8);
+  const KernelEntry* e = find_kernel(pl->type_a, pl->type_b, use->va, use->vb, kModeFp8, kFamBox);
   if (!e) return TTC_ERROR_UNSUPPORTED;
   const float fp8_max = pl->type_b == TTC_E4M3 ? 448.0f : 57344.0f;
   const float al = (float)alpha;
