@@ -2,8 +2,8 @@
 """Benchmark of the MI355X transposition path.
 
 Workload (one "step"): the 16 seeded random-permutation cases of rank 2..6,
-each at 200 MiB of A, in fp32 and in fp64, with beta = 0 (B not read,
-non-temporal stores) and beta = 1 (B read) — 64 out-of-place transpositions
+each at 200 MiB of A, in fp32 and in fp64, with beta = 0 (B not read) and
+beta = 1 (B read) — 64 out-of-place transpositions
 B[pi(i)] = alpha*A[i] + beta*B[pi(i)], inputs resident in HBM.  All beta = 0
 ops run first, then all beta = 1 ops, so consecutive ops never touch the same
 buffers and no op finds its data in the 256 MB last-level cache.
@@ -33,11 +33,13 @@ def log(*a):
 
 
 def kernel_name(info):
-    """rocprof-visible identity of the kernel a plan launches."""
-    ops = {(0, 0): "OpF32<float, float>", (1, 1): "OpDD"}
-    mode = "BETA" if not info["beta_is_zero"] else ("BZ_NT" if info["nt_store"] else "BZ")
-    return "transpose_kernel<%s, %d, %d, %s>" % (ops.get((info["type_a"], info["type_b"]), "?"), info["vec_a"],
-                                                  info["vec_b"], mode)
+    """The kernel a plan launches, spelled as rocprof shows its template
+    (ttc::<family kernel><Op, VA, VB, BETA, NT>)."""
+    ops = {(0, 0): "ttc::OpF32<float, float>", (1, 1): "ttc::OpDD"}
+    fam = "transpose_rc_kernel" if info["family"] == 2 else "transpose_kernel"
+    return "ttc::%s<%s, %d, %d, %s, %s>" % (fam, ops.get((info["type_a"], info["type_b"]), "?"), info["vec_a"],
+                                            info["vec_b"], "false" if info["beta_is_zero"] else "true",
+                                            "true" if info["nt_store"] else "false")
 
 
 def build_c2(torch, rt, wl, dev):

@@ -73,7 +73,7 @@ typedef struct {
   int vec_a;             /* elements per lane vector on the A (read) side; 0 = widest legal */
   int vec_b;             /* elements per lane vector on the B (write) side; 0 = widest legal */
   int wg_per_cu;         /* resident persistent work-groups per CU; 0 = occupancy limit */
-  int nt_store;          /* 0 = auto (on when beta_is_zero), 1 = on, 2 = off */
+  int nt_store;          /* non-temporal B stores (beta_is_zero only): 0 = auto (off), 1 = on, 2 = off */
   int grid_order;        /* 0 = auto (strided), 1 = strided tile walk, 2 = contiguous tile ranges */
   int run_a;             /* heuristic target run length on A's side, elements (0 = default) */
   int run_b;             /* heuristic target run length on B's side, elements (0 = default) */

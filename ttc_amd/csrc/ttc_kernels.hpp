@@ -190,6 +190,47 @@ template <class T, int N, bool NT> __device__ __forceinline__ void vstore(T* p, 
   else *reinterpret_cast<R*>(p) = r;
 }
 
+// ------------------------------------------------------------ buffer access
+// Buffer resources: the base address lives in scalar registers and each lane
+// supplies a 32-bit byte offset.  An offset at or above the record count
+// (2^31) fails the hardware range check: the load returns zeros and the store
+// is dropped, so edge lanes are predicated without branches and every memory
+// instruction is issued unconditionally (which keeps the compiler's vmcnt
+// accounting exact across the software pipeline).  0x00020000 is word 3 of a
+// raw (untyped, unswizzled) buffer resource on the gfx9 family.
+using BufRes = __amdgpu_buffer_rsrc_t;
+constexpr uint32_t kBufOOB = 0x80000000u;
+
+__device__ __forceinline__ BufRes make_res(const void* base) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)kBufOOB, 0x00020000);
+}
+
+template <class T, int N> __device__ __forceinline__ void bload(T (&dst)[N], BufRes r, uint32_t off) {
+  constexpr int kB = (int)sizeof(T) * N;
+  using R = typename Raw<kB>::T;
+  R v;
+  if constexpr (kB == 1) v = __builtin_amdgcn_raw_buffer_load_b8(r, off, 0, 0);
+  else if constexpr (kB == 2) v = __builtin_amdgcn_raw_buffer_load_b16(r, off, 0, 0);
+  else if constexpr (kB == 4) v = __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0);
+  else if constexpr (kB == 8) v = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
+  else v = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+  __builtin_memcpy(&dst[0], &v, sizeof(R));
+}
+
+// NT: non-temporal (streaming) store.
+template <class T, int N, bool NT> __device__ __forceinline__ void bstore(BufRes r, uint32_t off, const T (&src)[N]) {
+  constexpr int kB = (int)sizeof(T) * N;
+  constexpr int kAux = NT ? 2 : 0;
+  using R = typename Raw<kB>::T;
+  R v;
+  __builtin_memcpy(&v, &src[0], sizeof(R));
+  if constexpr (kB == 1) __builtin_amdgcn_raw_buffer_store_b8(v, r, off, 0, kAux);
+  else if constexpr (kB == 2) __builtin_amdgcn_raw_buffer_store_b16(v, r, off, 0, kAux);
+  else if constexpr (kB == 4) __builtin_amdgcn_raw_buffer_store_b32(v, r, off, 0, kAux);
+  else if constexpr (kB == 8) __builtin_amdgcn_raw_buffer_store_b64(v, r, off, 0, kAux);
+  else __builtin_amdgcn_raw_buffer_store_b128(v, r, off, 0, kAux);
+}
+
 // Where one tile of an op sits (host-precomputed for grouped launches):
 // element offsets of its origin in A and B and its extent along the split
 // dimensions (0xffff when the op has no such dimension).

@@ -100,7 +100,8 @@ struct RCParams {
   int32_t tsA, tsB;        // their tile extents
   uint32_t ntiles;
   int32_t grid_order;
-  int32_t lds_data_elems;  // RP * C
+  int32_t lds_data_elems;  // tile data region in LDS elements (RP * C, a spare vector slot, 16-byte rounding)
+  int32_t lds_spare;       // RP * C: where lanes past a partial tile stage their (unused) values
 };
 
 // Elements per thread per tile side: as many as fit 128 bytes, at most 32.

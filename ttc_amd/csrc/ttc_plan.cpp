@@ -466,9 +466,13 @@ int max_rc_volume(int ta, int tb) {
   return std::min(v, 32768 / lds_elem_bytes(ta, tb));
 }
 
+// Tile data (odd pitch RP: the write phase reads LDS with stride RP) plus a
+// spare vector slot, then the four offset tables.
+static long long rc_data_bytes(long long R, long long C, int lds_elem) {
+  return (((R | 1) * C + 16) * lds_elem + 15) / 16 * 16;
+}
 static long long rc_lds(long long R, long long C, int lds_elem) {
-  const long long RP = R | 1;  // odd pitch: the write phase reads LDS with stride RP
-  return (RP * C * lds_elem + 15) / 16 * 16 + (2 * R + 2 * C) * 4;
+  return rc_data_bytes(R, C, lds_elem) + (2 * R + 2 * C) * 4;
 }
 
 // Rows: A's leading dimensions 0, 1, ... up to B's stride-1 dimension;
@@ -525,7 +529,8 @@ ttc_status_t build_rc_params(const Geometry& g, const int* tile, int ta, int tb,
     if (ntiles > INT_MAX) return TTC_ERROR_UNSUPPORTED;
     p.gdiv[k] = make_fastdiv((uint32_t)gk);
   }
-  if (spanA * dtype_bytes(ta) >= INT_MAX || spanB * dtype_bytes(tb) >= INT_MAX) return TTC_ERROR_UNSUPPORTED;
+  // byte offsets (plus one vector) stay below the buffer range limit 2^31
+  if (spanA * dtype_bytes(ta) + 64 >= INT_MAX || spanB * dtype_bytes(tb) + 64 >= INT_MAX) return TTC_ERROR_UNSUPPORTED;
   const int p0 = g.perm[0];
   if (tile[0] % va || tile[p0] % vb || g.size[0] % va || g.size[p0] % vb) return TTC_ERROR_INVALID_VALUE;
   long long R = 1, C = 1;
@@ -571,8 +576,8 @@ ttc_status_t build_rc_params(const Geometry& g, const int* tile, int ta, int tb,
   p.rank = r;
   p.ntiles = (uint32_t)ntiles;
   p.grid_order = grid_order == 2 ? 1 : 0;
-  p.lds_data_elems = (int32_t)(((long long)p.RP * C * lds_elem_bytes(ta, tb) + 15) / 16 * 16 /
-                               lds_elem_bytes(ta, tb));
+  p.lds_data_elems = (int32_t)(rc_data_bytes(R, C, lds_elem_bytes(ta, tb)) / lds_elem_bytes(ta, tb));
+  p.lds_spare = (int32_t)(p.RP * C);
   lds_bytes = (int)lds;
   return TTC_SUCCESS;
 }
@@ -703,8 +708,10 @@ extern "C" ttc_status_t ttc_plan_create(ttc_plan_t* plan, ttc_dtype_t type_a, tt
     delete pl;
     return st;
   }
-  pl->nt_store = c.nt_store == 0 ? (pl->beta_is_zero ? 1 : 0) : (c.nt_store == 1 ? 1 : 0);
-  if (!pl->beta_is_zero) pl->nt_store = 0;  // B is read back: keep it cacheable
+  // Non-temporal stores measured slower than cached ones on MI355X for most
+  // shapes (the autotuner still tries them); never with beta != 0, where B
+  // is read back.
+  pl->nt_store = c.nt_store == 1 && pl->beta_is_zero ? 1 : 0;
   pl->grid_order = c.grid_order;
   pl->wg_per_cu = c.wg_per_cu;
   auto in_list = [](int w, int bytes) {

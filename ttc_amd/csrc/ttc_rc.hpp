@@ -12,7 +12,9 @@
 // Read phase: lane-consecutive vectors walk the tile row-major (i fastest:
 // contiguous in A).  Write phase: column-major (j fastest: contiguous in B).
 // The arithmetic (alpha in A's type before LDS, beta epilogue in B's type) is
-// the Op's, exactly as in the box kernel.
+// the Op's, exactly as in the box kernel.  Global memory is accessed through
+// buffer resources with 32-bit tile-relative byte offsets; lanes past a
+// partial tile get an out-of-range offset instead of a branch.
 #pragma once
 #include "ttc_kernels.hpp"
 
@@ -23,7 +25,6 @@ struct RCWorker {
   using TA = typename OP::TA;
   using TB = typename OP::TB;
   using L = typename OP::L;
-  static constexpr bool FP8 = is_fp8<OP>::value;
   static constexpr int NIA = rc_elems((int)sizeof(TA)) / VA;  // read vectors per thread per tile
   static constexpr int NIB = rc_elems((int)sizeof(TB)) / VB;  // write vectors per thread per tile
 
@@ -36,7 +37,7 @@ struct RCWorker {
 
   struct Coord {
     int64_t bA, bB;
-    uint32_t rows, cols;   // this tile's R / C
+    uint32_t rows, cols;   // this tile's R / C (rows * cols == 0: no tile, every access out of range)
     FastDiv dr, dc;        // division by rows / cols
   };
 
@@ -108,89 +109,107 @@ struct RCWorker {
     return c;
   }
 
+  // Read-side slot `it` of this thread: element (i, j) with i fastest.
+  // Lanes past the tile compute (0, 0) and are flagged invalid.
+  __device__ __forceinline__ bool slot_a(const Coord& c, int it, uint32_t& i, uint32_t& j) const {
+    const uint32_t l = (uint32_t)(it * kThreads + tid) * VA;
+    const bool ok = l < c.rows * c.cols;
+    const uint32_t lc = ok ? l : 0u;
+    j = fdiv(lc, c.dr);
+    i = lc - j * c.rows;
+    return ok;
+  }
+  // Write-side slot: element (i, j) with j fastest.
+  __device__ __forceinline__ bool slot_b(const Coord& c, int it, uint32_t& i, uint32_t& j) const {
+    const uint32_t l = (uint32_t)(it * kThreads + tid) * VB;
+    const bool ok = l < c.rows * c.cols;
+    const uint32_t lc = ok ? l : 0u;
+    i = fdiv(lc, c.dc);
+    j = lc - i * c.cols;
+    return ok;
+  }
+
   __device__ __forceinline__ void load(const RCParams& p, const Coord& c, const TA* __restrict__ A,
                                        TA (&va)[NIA][VA]) const {
-    const char* a = reinterpret_cast<const char*>(A + c.bA);
-    const uint32_t n = c.rows * c.cols;
+    const BufRes r = make_res(A + c.bA);
 #pragma unroll
     for (int it = 0; it < NIA; ++it) {
-      const uint32_t l = (uint32_t)(it * kThreads + tid) * VA;
-      if (l < n) {
-        const uint32_t j = fdiv(l, c.dr);
-        const uint32_t i = l - j * c.rows;
-        const uint32_t off = (uint32_t)((p.rowA_dense ? (int32_t)i : rowA[i]) + colA[j]) * (uint32_t)sizeof(TA);
-        vload<TA, VA>(va[it], reinterpret_cast<const TA*>(a + off));
-      }
+      uint32_t i, j;
+      const bool ok = slot_a(c, it, i, j);
+      const uint32_t off = (uint32_t)((p.rowA_dense ? (int32_t)i : rowA[i]) + colA[j]) * (uint32_t)sizeof(TA);
+      bload<TA, VA>(va[it], r, ok ? off : kBufOOB);
     }
   }
 
   __device__ __forceinline__ void stage(const RCParams& p, const Coord& c, const TA (&va)[NIA][VA],
-                                        typename OP::AL alpha, float& amax) const {
-    const uint32_t n = c.rows * c.cols;
+                                        typename OP::AL alpha) const {
 #pragma unroll
     for (int it = 0; it < NIA; ++it) {
-      const uint32_t l = (uint32_t)(it * kThreads + tid) * VA;
-      if (l < n) {
-        const uint32_t j = fdiv(l, c.dr);
-        const uint32_t i = l - j * c.rows;
-        const uint32_t li = i + j * (uint32_t)p.RP;
+      uint32_t i, j;
+      const bool ok = slot_a(c, it, i, j);
+      const uint32_t li = ok ? i + j * (uint32_t)p.RP : (uint32_t)p.lds_spare;
 #pragma unroll
-        for (int v = 0; v < VA; ++v) {
-          const L x = OP::prod(alpha, va[it][v]);
-          if constexpr (FP8) amax = fmaxf(amax, fabsf(x));
-          data[li + v] = x;
-        }
-      }
+      for (int v = 0; v < VA; ++v) data[li + v] = OP::prod(alpha, va[it][v]);
+    }
+  }
+
+  __device__ __forceinline__ uint32_t off_b(const RCParams& p, uint32_t i, uint32_t j) const {
+    return (uint32_t)(rowB[i] + (p.colB_dense ? (int32_t)j : colB[j])) * (uint32_t)sizeof(TB);
+  }
+
+  // Old values of B for the beta epilogue.
+  __device__ __forceinline__ void load_b(const RCParams& p, const Coord& c, const TB* __restrict__ B,
+                                         TB (&ob)[NIB][VB]) const {
+    const BufRes r = make_res(B + c.bB);
+#pragma unroll
+    for (int it = 0; it < NIB; ++it) {
+      uint32_t i, j;
+      const bool ok = slot_b(c, it, i, j);
+      bload<TB, VB>(ob[it], r, ok ? off_b(p, i, j) : kBufOOB);
     }
   }
 
   __device__ __forceinline__ void store(const RCParams& p, const Coord& c, TB* __restrict__ B,
-                                        typename OP::BE beta, float scale) const {
-    char* b = reinterpret_cast<char*>(B + c.bB);
-    const uint32_t n = c.rows * c.cols;
+                                        typename OP::BE beta, const TB (&ob)[NIB][VB]) const {
+    const BufRes r = make_res(B + c.bB);
     const uint32_t RP = (uint32_t)p.RP;
 #pragma unroll
     for (int it = 0; it < NIB; ++it) {
-      const uint32_t l = (uint32_t)(it * kThreads + tid) * VB;
-      if (l < n) {
-        const uint32_t i = fdiv(l, c.dc);
-        const uint32_t j = l - i * c.cols;
-        const uint32_t off = (uint32_t)(rowB[i] + (p.colB_dense ? (int32_t)j : colB[j])) * (uint32_t)sizeof(TB);
-        const uint32_t li = i + j * RP;
-        TB o[VB];
-        if constexpr (FP8) {
+      uint32_t i, j;
+      const bool ok = slot_b(c, it, i, j);
+      const uint32_t li = i + j * RP;
+      TB o[VB];
 #pragma unroll
-          for (int v = 0; v < VB; ++v) o[v] = OP::outs(data[li + v * RP], scale);
-        } else if constexpr (BETA) {
-          TB ob[VB];
-          vload<TB, VB>(ob, reinterpret_cast<const TB*>(b + off));
-#pragma unroll
-          for (int v = 0; v < VB; ++v) o[v] = OP::outb(data[li + v * RP], beta, ob[v]);
-        } else {
-#pragma unroll
-          for (int v = 0; v < VB; ++v) o[v] = OP::out(data[li + v * RP]);
-        }
-        vstore<TB, VB, NT>(reinterpret_cast<TB*>(b + off), o);
+      for (int v = 0; v < VB; ++v) {
+        if constexpr (BETA) o[v] = OP::outb(data[li + v * RP], beta, ob[it][v]);
+        else o[v] = OP::out(data[li + v * RP]);
       }
+      bstore<TB, VB, NT>(r, ok ? off_b(p, i, j) : kBufOOB, o);
     }
   }
 
-  // Software-pipelined walk, as the box kernel's.
+  // Software pipeline over this work-group's tiles: the next tile's reads
+  // are in flight while the current tile is written.  Memory operations per
+  // iteration, in issue order: old B of tile t (beta only), A of tile t+1,
+  // B stores of tile t; staging tile t+1 then waits only for its own reads.
+  // The last iteration prefetches an empty tile (all lanes out of range).
   __device__ __forceinline__ void run(const RCParams& p, const TA* __restrict__ A, TB* __restrict__ B,
                                       uint32_t first, uint32_t end, uint32_t step, typename OP::AL alpha,
-                                      typename OP::BE beta, float scale, float& amax) {
+                                      typename OP::BE beta) {
     if (first >= end) return;
     TA va[NIA][VA];
+    TB ob[NIB][VB];
     Coord c = coord(p, first);
     load(p, c, A, va);
     for (uint32_t t = first; t < end; t += step) {
-      stage(p, c, va, alpha, amax);
+      stage(p, c, va, alpha);
       __syncthreads();
+      if constexpr (BETA) load_b(p, c, B, ob);
       const uint32_t tn = t + step;
-      const bool more = tn < end;
-      const Coord cn = coord(p, more ? tn : t);
-      if (more) load(p, cn, A, va);
-      store(p, c, B, beta, scale);
+      Coord cn = coord(p, tn < end ? tn : t);
+      if (tn >= end) cn.rows = 0;
+      load(p, cn, A, va);
+      store(p, c, B, beta, ob);
       __syncthreads();
       c = cn;
     }
@@ -205,8 +224,7 @@ __global__ void __launch_bounds__(kThreads) transpose_rc_kernel(const RCParams p
   RCWorker<OP, VA, VB, BETA, NT> w;
   w.setup(p, lds_raw, threadIdx.x);
   const TileRange r = tile_range(p.ntiles, blockIdx.x, gridDim.x, p.grid_order);
-  float amax = 0.0f;
-  w.run(p, A, B, r.first, r.end, r.step, (typename OP::AL)alpha, (typename OP::BE)beta, 1.0f, amax);
+  w.run(p, A, B, r.first, r.end, r.step, (typename OP::AL)alpha, (typename OP::BE)beta);
 }
 
 }  // namespace ttc
