@@ -74,10 +74,12 @@ typedef struct {
   int vec_b;             /* elements per lane vector on the B (write) side; 0 = widest legal */
   int wg_per_cu;         /* resident persistent work-groups per CU; 0 = occupancy limit */
   int nt_store;          /* non-temporal B stores (beta_is_zero only): 0 = auto (off), 1 = on, 2 = off */
-  int grid_order;        /* 0 = auto (strided), 1 = strided tile walk, 2 = contiguous tile ranges */
+  int grid_order;        /* 0 = auto (3), 1 = strided tile walk, 2 = contiguous tile ranges,
+                            3 = strided walk with the work-groups of one XCD on consecutive tiles */
   int run_a;             /* heuristic target run length on A's side, elements (0 = default) */
   int run_b;             /* heuristic target run length on B's side, elements (0 = default) */
   int family;            /* kernel family: 0 = auto, 1 = box tile, 2 = row x column tile */
+  int tile_order;        /* tile numbering (row x column tiles): 0 = auto (B order), 1 = A order, 2 = B order */
 } ttc_config_t;
 
 /* What a plan resolved to (fused geometry, tile, launch shape). */
@@ -105,6 +107,7 @@ typedef struct {
   int run_a, run_b;              /* contiguous element runs of a full tile in A and B */
   int family;                    /* 1 = box tile kernel, 2 = row x column tile kernel */
   int rows, cols;                /* row x column tile: row and column counts (0 for box tiles) */
+  int tile_order;                /* 1 = A order, 2 = B order */
 } ttc_plan_info_t;
 
 /* Library version string. */

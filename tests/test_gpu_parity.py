@@ -90,9 +90,10 @@ def test_random_shapes(gpu, code):
         for fam in families(tA, tB, perm, size):
             n_rc += fam == 2
             for beta in (0.0, 0.7):
+                order = 1 + (i + int(beta > 0)) % 2  # row x column tiles: A- and B-order numbering
                 info = check(gpu, tA, tB, perm, size, 1.3, beta, seed=100 + i, dist="outliers",
-                             config=dict(family=fam))
-                assert info["family"] == fam
+                             config=dict(family=fam, tile_order=order))
+                assert info["family"] == fam and info["tile_order"] == (order if fam == 2 else 1)
     assert n_rc >= 9
 
 
@@ -124,7 +125,7 @@ def test_every_vector_width_and_mode(gpu):
         for va in vec_widths(rt.DTYPE_BYTES[tA]):
             for vb in vec_widths(rt.DTYPE_BYTES[tB]):
                 for nt in (1, 2):
-                    for order in (1, 2):
+                    for order in (1, 2, 3):
                         for fam in (1, 2):
                             cfg = dict(vec_a=va, vec_b=vb, nt_store=nt, grid_order=order, wg_per_cu=1,
                                        family=fam)
@@ -152,7 +153,7 @@ def test_explicit_row_column_tiles(gpu):
     for perm, size, tile in cases:
         for code in ("s", "d", "z", "bs"):
             tA, tB = CODE[code]
-            cfg = dict(tile=tile, family=2, vec_a=1, vec_b=1)
+            cfg = dict(tile=tile, family=2, vec_a=1, vec_b=1, tile_order=1 + n % 2)
             # volume limit of the family: 256 threads x (128 bytes, at most 32 elements) each side
             limit = 256 * min(min(32, 128 // rt.DTYPE_BYTES[t]) for t in (tA, tB))
             if int(np.prod(tile)) > limit:

@@ -480,9 +480,18 @@ struct TileWorker {
 struct TileRange {
   uint32_t first, end, step;
 };
+// Work-groups are dispatched round-robin over the 8 XCDs (work-group w runs
+// on XCD w % 8), each XCD with its own L2.  order 2 renumbers them so that
+// the work-groups of one XCD take consecutive tiles of each round of the
+// strided walk: neighbouring tiles share the cache lines at their edges, and
+// those then meet in one L2 instead of being fetched (and partially written
+// back) by two.
+constexpr uint32_t kXcds = 8;
+
 __device__ __forceinline__ TileRange tile_range(uint32_t ntiles, uint32_t wg, uint32_t nwg, int order) {
   TileRange r;
-  if (order == 0) {
+  if (order == 0 || order == 2) {
+    if (order == 2 && nwg % kXcds == 0) wg = (wg % kXcds) * (nwg / kXcds) + wg / kXcds;
     r.first = wg; r.end = ntiles; r.step = nwg;
   } else {
     const uint32_t per = ntiles / nwg, extra = ntiles % nwg;

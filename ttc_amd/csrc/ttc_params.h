@@ -81,23 +81,23 @@ struct RCParams {
   int32_t R, C;          // rows (A-group volume) and columns (B-group volume) of a full tile
   int32_t Rl, Cl;        // rows / columns of the last tile along the split dims (== R / C if none)
   FastDiv dR, dRl, dC, dCl;
-  int32_t RP;            // LDS pitch of a column (>= R)
-  int32_t rowA_dense;    // rowA[i] == i (read vectors need this)
-  int32_t colB_dense;    // colB[j] == j (write vectors need this)
+  int32_t RP;            // LDS pitch of a column (>= R, odd)
+  int32_t rowA_dense;    // rowA[i] == i * sizeof(TA): the rows are one contiguous run of A
+  int32_t colB_dense;    // colB[j] == j * sizeof(TB): the columns are one contiguous run of B
   int32_t na, nb;        // dimensions in the A group / B group
   int32_t ta[kMaxDim];   // tile extents of the A group, A order
   FastDiv dta[kMaxDim];
-  int32_t saA[kMaxDim], sbA[kMaxDim];  // A and B strides of the A-group dims
+  int32_t saA[kMaxDim], sbA[kMaxDim];  // A and B byte strides of the A-group dims
   int32_t tb[kMaxDim];   // tile extents of the B group, B order
   FastDiv dtb[kMaxDim];
-  int32_t saB[kMaxDim], sbB[kMaxDim];  // A and B strides of the B-group dims
-  // tile grid over all dimensions (A order)
-  int32_t rank;
+  int32_t saB[kMaxDim], sbB[kMaxDim];  // A and B byte strides of the B-group dims
+  // Tile grid: only the dimensions with more than one tile, in walk order
+  // (A order or B order); position g describes one such dimension.
+  int32_t grank;
   int32_t n[kMaxDim], t[kMaxDim];
-  FastDiv gdiv[kMaxDim];
-  int64_t tsa[kMaxDim], tsb[kMaxDim];
-  int32_t splitA, splitB;  // the partial dim of the A group / B group (-1 none)
-  int32_t tsA, tsB;        // their tile extents
+  FastDiv gdiv[kMaxDim];   // division by the number of tiles along the dimension
+  int64_t tsa[kMaxDim], tsb[kMaxDim];  // element offset of the next tile along it, in A and B
+  int32_t splitA, splitB;  // grid positions of the row group's / column group's partial dimension (-1 none)
   uint32_t ntiles;
   int32_t grid_order;
   int32_t lds_data_elems;  // tile data region in LDS elements (RP * C, a spare vector slot, 16-byte rounding)

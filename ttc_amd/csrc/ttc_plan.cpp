@@ -364,6 +364,16 @@ void choose_tile(const Geometry& g, int ta, int tb, int va, int vb, int run_a, i
 }
 
 // ------------------------------------------------------------ launch params
+// ttc_config_t.grid_order -> the kernels' tile_range order: strided walk (0),
+// contiguous ranges (1), strided walk with XCD-grouped work-groups (2).
+int kernel_grid_order(int cfg) {
+  switch (cfg) {
+    case 1: return 0;
+    case 2: return 1;
+    default: return 2;
+  }
+}
+
 ttc_status_t build_params(const Geometry& g, const int* tile, int ta, int tb, int va, int vb, int grid_order,
                           TileParams& p, int& lds_bytes, int& lds_pad) {
   const int lds_elem = lds_elem_bytes(ta, tb);
@@ -453,7 +463,7 @@ ttc_status_t build_params(const Geometry& g, const int* tile, int ta, int tb, in
   p.ts1 = make_fastdiv(p.split1 >= 0 ? (uint32_t)tile[p.split1] : 1);
   lds_bytes = (int)((elems * lds_elem + 15) / 16 * 16);
   if (lds_bytes > 65536) return TTC_ERROR_UNSUPPORTED;
-  p.grid_order = grid_order == 2 ? 1 : 0;
+  p.grid_order = kernel_grid_order(grid_order);
   return TTC_SUCCESS;
 }
 
@@ -508,29 +518,26 @@ bool rc_groups(const Geometry& g, const int* tile, int& na, int* ga, int& nb, in
 }
 
 ttc_status_t build_rc_params(const Geometry& g, const int* tile, int ta, int tb, int va, int vb, int grid_order,
-                             RCParams& p, int& lds_bytes) {
+                             int tile_order, RCParams& p, int& lds_bytes) {
   int na, nb, ga[kMaxDim], gb[kMaxDim];
   if (!rc_groups(g, tile, na, ga, nb, gb)) return TTC_ERROR_INVALID_VALUE;
   std::memset(&p, 0, sizeof(p));
   const int r = g.rank;
+  const int szA = dtype_bytes(ta), szB = dtype_bytes(tb);
   long long ntiles = 1, spanA = 0, spanB = 0;
+  long long gk[kMaxDim];
   for (int k = 0; k < r; ++k) {
     const int n = g.size[k], t = tile[k];
     if (t < 1 || (n > 0 && t > n)) return TTC_ERROR_INVALID_VALUE;
     spanA += (long long)(t - 1) * g.sa[k];
     spanB += (long long)(t - 1) * g.sb[k];
-    p.n[k] = n;
-    p.t[k] = t;
-    p.tsa[k] = (int64_t)t * g.sa[k];
-    p.tsb[k] = (int64_t)t * g.sb[k];
-    const long long gk = n > 0 ? (n + t - 1) / t : 1;
+    gk[k] = n > 0 ? (n + t - 1) / t : 1;
     if (n == 0) ntiles = 0;
-    ntiles *= gk;
+    ntiles *= gk[k];
     if (ntiles > INT_MAX) return TTC_ERROR_UNSUPPORTED;
-    p.gdiv[k] = make_fastdiv((uint32_t)gk);
   }
   // byte offsets (plus one vector) stay below the buffer range limit 2^31
-  if (spanA * dtype_bytes(ta) + 64 >= INT_MAX || spanB * dtype_bytes(tb) + 64 >= INT_MAX) return TTC_ERROR_UNSUPPORTED;
+  if (spanA * szA + 64 >= INT_MAX || spanB * szB + 64 >= INT_MAX) return TTC_ERROR_UNSUPPORTED;
   const int p0 = g.perm[0];
   if (tile[0] % va || tile[p0] % vb || g.size[0] % va || g.size[p0] % vb) return TTC_ERROR_INVALID_VALUE;
   long long R = 1, C = 1;
@@ -541,8 +548,8 @@ ttc_status_t build_rc_params(const Geometry& g, const int* tile, int ta, int tb,
     if (g.sa[k] != R) p.rowA_dense = 0;
     p.ta[m] = tile[k];
     p.dta[m] = make_fastdiv((uint32_t)tile[k]);
-    p.saA[m] = (int32_t)g.sa[k];
-    p.sbA[m] = (int32_t)g.sb[k];
+    p.saA[m] = (int32_t)(g.sa[k] * szA);
+    p.sbA[m] = (int32_t)(g.sb[k] * szB);
     R *= tile[k];
   }
   p.colB_dense = 1;
@@ -552,30 +559,42 @@ ttc_status_t build_rc_params(const Geometry& g, const int* tile, int ta, int tb,
     if (g.sb[k] != C) p.colB_dense = 0;
     p.tb[m] = tile[k];
     p.dtb[m] = make_fastdiv((uint32_t)tile[k]);
-    p.saB[m] = (int32_t)g.sa[k];
-    p.sbB[m] = (int32_t)g.sb[k];
+    p.saB[m] = (int32_t)(g.sa[k] * szA);
+    p.sbB[m] = (int32_t)(g.sb[k] * szB);
     C *= tile[k];
   }
   if (R * C > max_rc_volume(ta, tb)) return TTC_ERROR_UNSUPPORTED;
   const long long lds = rc_lds(R, C, lds_elem_bytes(ta, tb));
   if (lds > kRcLdsBudget) return TTC_ERROR_UNSUPPORTED;
   const int ka = ga[na - 1], kb = gb[nb - 1];
-  p.splitA = g.size[ka] > 0 && g.size[ka] % tile[ka] ? ka : -1;
-  p.splitB = g.size[kb] > 0 && g.size[kb] % tile[kb] ? kb : -1;
-  p.tsA = tile[ka];
-  p.tsB = tile[kb];
+  const bool partA = g.size[ka] > 0 && g.size[ka] % tile[ka];
+  const bool partB = g.size[kb] > 0 && g.size[kb] % tile[kb];
   p.R = (int32_t)R;
   p.C = (int32_t)C;
-  p.Rl = p.splitA >= 0 ? (int32_t)(R / tile[ka] * (g.size[ka] % tile[ka])) : p.R;
-  p.Cl = p.splitB >= 0 ? (int32_t)(C / tile[kb] * (g.size[kb] % tile[kb])) : p.C;
+  p.Rl = partA ? (int32_t)(R / tile[ka] * (g.size[ka] % tile[ka])) : p.R;
+  p.Cl = partB ? (int32_t)(C / tile[kb] * (g.size[kb] % tile[kb])) : p.C;
   p.dR = make_fastdiv((uint32_t)p.R);
   p.dRl = make_fastdiv((uint32_t)p.Rl);
   p.dC = make_fastdiv((uint32_t)p.C);
   p.dCl = make_fastdiv((uint32_t)p.Cl);
   p.RP = (int32_t)(R | 1);
-  p.rank = r;
+  // tile grid: dimensions with more than one tile, in A or B order
+  p.splitA = p.splitB = -1;
+  p.grank = 0;
+  for (int q = 0; q < r; ++q) {
+    const int k = tile_order == 2 ? g.perm[q] : q;
+    if (gk[k] <= 1) continue;
+    const int pos = p.grank++;
+    p.n[pos] = g.size[k];
+    p.t[pos] = tile[k];
+    p.gdiv[pos] = make_fastdiv((uint32_t)gk[k]);
+    p.tsa[pos] = (int64_t)tile[k] * g.sa[k];
+    p.tsb[pos] = (int64_t)tile[k] * g.sb[k];
+    if (partA && k == ka) p.splitA = pos;
+    if (partB && k == kb) p.splitB = pos;
+  }
   p.ntiles = (uint32_t)ntiles;
-  p.grid_order = grid_order == 2 ? 1 : 0;
+  p.grid_order = kernel_grid_order(grid_order);
   p.lds_data_elems = (int32_t)(rc_data_bytes(R, C, lds_elem_bytes(ta, tb)) / lds_elem_bytes(ta, tb));
   p.lds_spare = (int32_t)(p.RP * C);
   lds_bytes = (int)lds;
@@ -595,7 +614,7 @@ bool choose_rc_tile(const Geometry& g, int ta, int tb, int va, int vb, int run_a
     for (int rb = 1; rb <= vmax; rb *= 2) {
       if ((long long)ra * rb > 4LL * vmax) continue;
       tile_for_runs(g, va, vb, vmax, szA, szB, run_a > 0 ? run_a : ra, run_b > 0 ? run_b : rb, cand);
-      if (build_rc_params(g, cand, ta, tb, va, vb, 0, p, lds) != TTC_SUCCESS) continue;
+      if (build_rc_params(g, cand, ta, tb, va, vb, 0, 1, p, lds) != TTC_SUCCESS) continue;
       const double sc = tile_score(g, cand, szA, szB);
       if (sc > best * (1.0 + 1e-9)) {
         best = sc;
@@ -666,7 +685,8 @@ static ttc_status_t plan_fill(ttc_plan_s* pl, int va, int vb, const int* tile_in
   ttc_status_t st;
   if (fam == kFamilyRC) {
     std::memset(&pl->p, 0, sizeof(pl->p));
-    st = build_rc_params(g, tile, pl->type_a, pl->type_b, va, vb, pl->grid_order, pl->rc, pl->lds_bytes);
+    st = build_rc_params(g, tile, pl->type_a, pl->type_b, va, vb, pl->grid_order, pl->tile_order, pl->rc,
+                         pl->lds_bytes);
     pl->lds_pad = pl->rc.RP != pl->rc.R;
     pl->p.ntiles = pl->rc.ntiles;
     pl->p.V = pl->rc.R * pl->rc.C;
@@ -694,8 +714,8 @@ extern "C" ttc_status_t ttc_plan_create(ttc_plan_t* plan, ttc_dtype_t type_a, tt
   ttc_config_t c;
   std::memset(&c, 0, sizeof(c));
   if (cfg) c = *cfg;
-  if (c.nt_store < 0 || c.nt_store > 2 || c.grid_order < 0 || c.grid_order > 2 || c.wg_per_cu < 0 || c.vec_a < 0 ||
-      c.vec_b < 0 || c.family < 0 || c.family > 2)
+  if (c.nt_store < 0 || c.nt_store > 2 || c.grid_order < 0 || c.grid_order > 3 || c.wg_per_cu < 0 || c.vec_a < 0 ||
+      c.vec_b < 0 || c.family < 0 || c.family > 2 || c.tile_order < 0 || c.tile_order > 2)
     return TTC_ERROR_INVALID_VALUE;
   ttc_plan_s* pl = new (std::nothrow) ttc_plan_s();
   if (!pl) return TTC_ERROR_ALLOC;
@@ -713,6 +733,9 @@ extern "C" ttc_status_t ttc_plan_create(ttc_plan_t* plan, ttc_dtype_t type_a, tt
   // is read back.
   pl->nt_store = c.nt_store == 1 && pl->beta_is_zero ? 1 : 0;
   pl->grid_order = c.grid_order;
+  // B-order numbering: consecutive tiles of a round share B's cache lines
+  // (measured the better default together with XCD-grouped work-groups)
+  pl->tile_order = c.tile_order == 0 ? 2 : c.tile_order;
   pl->wg_per_cu = c.wg_per_cu;
   auto in_list = [](int w, int bytes) {
     for (int x : vec_widths(bytes))
@@ -777,7 +800,7 @@ extern "C" ttc_status_t ttc_plan_get_info(ttc_plan_t pl, ttc_plan_info_t* info) 
   info->lds_bytes = pl->lds_bytes;
   info->lds_pad = pl->lds_pad;
   info->nt_store = pl->nt_store;
-  info->grid_order = pl->p.grid_order;
+  info->grid_order = pl->p.grid_order == 0 ? 1 : (pl->p.grid_order == 1 ? 2 : 3);  // config numbering
   info->wg_per_cu = pl->wg_per_cu;
   info->elements = g.elements;
   const int sa = dtype_bytes(pl->type_a), sb = dtype_bytes(pl->type_b);
@@ -788,6 +811,7 @@ extern "C" ttc_status_t ttc_plan_get_info(ttc_plan_t pl, ttc_plan_info_t* info) 
   info->run_a = pl->run_a;
   info->run_b = pl->run_b;
   info->family = pl->family;
+  info->tile_order = pl->family == kFamilyRC ? pl->tile_order : 1;
   if (pl->family == kFamilyRC) {
     info->slots_a = rc_elems(sa) / pl->va;
     info->slots_b = rc_elems(sb) / pl->vb;

@@ -54,6 +54,8 @@ struct PlanChoice {
   int run_a, run_b;
 };
 
+int kernel_grid_order(int cfg);
+
 // Fills TileParams for a geometry and a tile; returns TTC_ERROR_UNSUPPORTED
 // when the tile violates a kernel limit.
 ttc_status_t build_params(const Geometry& g, const int* tile, int ta, int tb, int va, int vb, int grid_order,
@@ -78,7 +80,7 @@ int max_tile_volume(int ta, int tb, int va, int vb);
 int max_rc_volume(int ta, int tb);
 bool rc_groups(const Geometry& g, const int* tile, int& na, int* ga, int& nb, int* gb);
 ttc_status_t build_rc_params(const Geometry& g, const int* tile, int ta, int tb, int va, int vb, int grid_order,
-                             RCParams& p, int& lds_bytes);
+                             int tile_order, RCParams& p, int& lds_bytes);
 bool choose_rc_tile(const Geometry& g, int ta, int tb, int va, int vb, int run_a, int run_b, int* tile,
                     double* score);
 
@@ -96,6 +98,7 @@ struct ttc_plan_s {
   int lds_bytes, lds_pad;
   int run_a, run_b;
   int family;            // 1 box tile kernel (p), 2 row x column kernel (rc)
+  int tile_order;        // 1 A order, 2 B order (row x column tiles only)
   ttc::RCParams rc;
   ttc_plan_s* fallback;  // scalar-vector plan for misaligned pointers (may be null)
 };

@@ -20,6 +20,17 @@
 
 namespace ttc {
 
+// The kernel parameters seen through the kernel-argument segment (constant
+// address space: loads from it are scalar and uniform).
+using KParams = const __attribute__((address_space(4))) RCParams;
+
+__device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ int64_t uniform64(int64_t v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)(uint64_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)v >> 32));
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
 template <class OP, int VA, int VB, bool BETA, bool NT>
 struct RCWorker {
   using TA = typename OP::TA;
@@ -27,22 +38,32 @@ struct RCWorker {
   using L = typename OP::L;
   static constexpr int NIA = rc_elems((int)sizeof(TA)) / VA;  // read vectors per thread per tile
   static constexpr int NIB = rc_elems((int)sizeof(TB)) / VB;  // write vectors per thread per tile
+  static constexpr uint32_t SA = kThreads * VA;               // read-walk step between a thread's slots
+  static constexpr uint32_t SB = kThreads * VB;               // write-walk step
 
-  L* data;             // LDS tile
-  const int32_t* rowA; // LDS tables
+  L* data;              // LDS tile
+  const int32_t* rowA;  // LDS tables, byte offsets
   const int32_t* rowB;
   const int32_t* colA;
   const int32_t* colB;
-  int tid;
+  uint32_t tid;
 
-  struct Coord {
-    int64_t bA, bB;
-    uint32_t rows, cols;   // this tile's R / C (rows * cols == 0: no tile, every access out of range)
-    FastDiv dr, dc;        // division by rows / cols
+  // One tile: origin, shape, and each walk's start and per-slot step.  The
+  // read walk visits (i, j) with i fastest, the write walk with j fastest;
+  // slot `it` of a thread is element l = tid * V + it * S, and stepping by S
+  // is an add with one wrap ((i, j) never needs a division after the start).
+  struct Tile {
+    int64_t bA, bB;        // element offsets of the tile origin
+    uint32_t rows, cols;
+    int32_t limA, limB;    // slot it is inside the tile iff it * S < lim (per lane)
+    uint32_t ai, aj;       // read walk start (per lane)
+    uint32_t adi, adj;     // SA = adj * rows + adi
+    uint32_t bi, bj;       // write walk start (per lane)
+    uint32_t bdi, bdj;     // SB = bdi * cols + bdj
   };
 
   // Offset tables into LDS (after the data region), once per launch.
-  __device__ __forceinline__ void setup(const RCParams& p, unsigned char* lds_raw, int tid_) {
+  __device__ __forceinline__ void setup(const RCParams& p, unsigned char* lds_raw, uint32_t tid_) {
     tid = tid_;
     data = reinterpret_cast<L*>(lds_raw);
     int32_t* tab = reinterpret_cast<int32_t*>(lds_raw + (size_t)p.lds_data_elems * sizeof(L));
@@ -50,7 +71,7 @@ struct RCWorker {
     int32_t* rB = tab + p.R;
     int32_t* cA = tab + 2 * p.R;
     int32_t* cB = tab + 2 * p.R + p.C;
-    for (int i = tid; i < p.R; i += kThreads) {
+    for (int i = (int)tid; i < p.R; i += kThreads) {
       uint32_t l = (uint32_t)i;
       int32_t oa = 0, ob = 0;
       for (int m = 0; m < p.na; ++m) {
@@ -63,7 +84,7 @@ struct RCWorker {
       rA[i] = oa;
       rB[i] = ob;
     }
-    for (int j = tid; j < p.C; j += kThreads) {
+    for (int j = (int)tid; j < p.C; j += kThreads) {
       uint32_t l = (uint32_t)j;
       int32_t oa = 0, ob = 0;
       for (int m = 0; m < p.nb; ++m) {
@@ -83,109 +104,146 @@ struct RCWorker {
     __syncthreads();
   }
 
-  __device__ __forceinline__ Coord coord(const RCParams& p, uint32_t tile_id) const {
-    Coord c;
-    uint32_t rem = tile_id;
+  // Tile `id` of the grid (kp: the parameters read from the kernel-argument
+  // segment with a runtime index, so that the grid arrays are not pinned in
+  // scalar registers across the tile loop).  empty: a tile with no elements.
+  __device__ __forceinline__ Tile tile(const RCParams& p, KParams* kp, uint32_t id, bool empty) const {
+    Tile c;
+    uint32_t rem = id;
     c.bA = 0;
     c.bB = 0;
     bool lastA = false, lastB = false;
-#pragma unroll
-    for (int k = 0; k < kMaxDim; ++k) {
-      if (k < p.rank) {
-        const uint32_t q = fdiv(rem, p.gdiv[k]);
-        const uint32_t x = rem - q * p.gdiv[k].d;
-        rem = q;
-        c.bA += (int64_t)x * p.tsa[k];
-        c.bB += (int64_t)x * p.tsb[k];
-        const bool partial = (uint32_t)p.n[k] - x * (uint32_t)p.t[k] < (uint32_t)p.t[k];
-        if (k == p.splitA) lastA = partial;
-        if (k == p.splitB) lastB = partial;
-      }
+    const int gr = p.grank;
+#pragma unroll 1
+    for (int g = 0; g < gr; ++g) {
+      FastDiv d;  // field by field: no struct copy across address spaces
+      d.d = kp->gdiv[g].d;
+      d.m = kp->gdiv[g].m;
+      d.s = kp->gdiv[g].s;
+      const uint32_t q = fdiv(rem, d);
+      const uint32_t x = rem - q * d.d;
+      rem = q;
+      c.bA += (int64_t)x * kp->tsa[g];
+      c.bB += (int64_t)x * kp->tsb[g];
+      const bool partial = (uint32_t)kp->n[g] - x * (uint32_t)kp->t[g] < (uint32_t)kp->t[g];
+      lastA |= g == p.splitA && partial;
+      lastB |= g == p.splitB && partial;
     }
+    c.bA = uniform64(c.bA);
+    c.bB = uniform64(c.bB);
     c.rows = lastA ? (uint32_t)p.Rl : (uint32_t)p.R;
     c.cols = lastB ? (uint32_t)p.Cl : (uint32_t)p.C;
-    c.dr = lastA ? p.dRl : p.dR;
-    c.dc = lastB ? p.dCl : p.dC;
+    const FastDiv dr = lastA ? p.dRl : p.dR;
+    const FastDiv dc = lastB ? p.dCl : p.dC;
+    const int32_t n = empty ? 0 : (int32_t)(c.rows * c.cols);
+    c.limA = n - (int32_t)(tid * VA);
+    c.limB = n - (int32_t)(tid * VB);
+    c.aj = fdiv(tid * VA, dr);
+    c.ai = tid * VA - c.aj * c.rows;
+    c.adj = fdiv(SA, dr);
+    c.adi = SA - c.adj * c.rows;
+    c.bi = fdiv(tid * VB, dc);
+    c.bj = tid * VB - c.bi * c.cols;
+    c.bdi = fdiv(SB, dc);
+    c.bdj = SB - c.bdi * c.cols;
     return c;
   }
 
-  // Read-side slot `it` of this thread: element (i, j) with i fastest.
-  // Lanes past the tile compute (0, 0) and are flagged invalid.
-  __device__ __forceinline__ bool slot_a(const Coord& c, int it, uint32_t& i, uint32_t& j) const {
-    const uint32_t l = (uint32_t)(it * kThreads + tid) * VA;
-    const bool ok = l < c.rows * c.cols;
-    const uint32_t lc = ok ? l : 0u;
-    j = fdiv(lc, c.dr);
-    i = lc - j * c.rows;
-    return ok;
-  }
-  // Write-side slot: element (i, j) with j fastest.
-  __device__ __forceinline__ bool slot_b(const Coord& c, int it, uint32_t& i, uint32_t& j) const {
-    const uint32_t l = (uint32_t)(it * kThreads + tid) * VB;
-    const bool ok = l < c.rows * c.cols;
-    const uint32_t lc = ok ? l : 0u;
-    i = fdiv(lc, c.dc);
-    j = lc - i * c.cols;
-    return ok;
-  }
-
-  __device__ __forceinline__ void load(const RCParams& p, const Coord& c, const TA* __restrict__ A,
-                                       TA (&va)[NIA][VA]) const {
+  template <bool DENSE>
+  __device__ __forceinline__ void load_impl(const RCParams& p, const Tile& c, const TA* __restrict__ A,
+                                            TA (&va)[NIA][VA]) const {
     const BufRes r = make_res(A + c.bA);
+    const uint32_t cmax = (uint32_t)p.C - 1;
+    uint32_t i = c.ai, j = c.aj;
 #pragma unroll
     for (int it = 0; it < NIA; ++it) {
-      uint32_t i, j;
-      const bool ok = slot_a(c, it, i, j);
-      const uint32_t off = (uint32_t)((p.rowA_dense ? (int32_t)i : rowA[i]) + colA[j]) * (uint32_t)sizeof(TA);
+      const bool ok = (int32_t)(it * SA) < c.limA;
+      const uint32_t jc = j < cmax ? j : cmax;  // past the tile: any table entry
+      const uint32_t off = (DENSE ? i * (uint32_t)sizeof(TA) : (uint32_t)rowA[i]) + (uint32_t)colA[jc];
       bload<TA, VA>(va[it], r, ok ? off : kBufOOB);
+      i += c.adi;
+      j += c.adj;
+      if (i >= c.rows) {
+        i -= c.rows;
+        ++j;
+      }
     }
   }
+  __device__ __forceinline__ void load(const RCParams& p, const Tile& c, const TA* __restrict__ A,
+                                       TA (&va)[NIA][VA]) const {
+    if (p.rowA_dense) load_impl<true>(p, c, A, va);
+    else load_impl<false>(p, c, A, va);
+  }
 
-  __device__ __forceinline__ void stage(const RCParams& p, const Coord& c, const TA (&va)[NIA][VA],
+  __device__ __forceinline__ void stage(const RCParams& p, const Tile& c, const TA (&va)[NIA][VA],
                                         typename OP::AL alpha) const {
+    const uint32_t RP = (uint32_t)p.RP;
+    const uint32_t dli = c.adi + c.adj * RP, wrap = RP - c.rows;
+    uint32_t i = c.ai, li = c.ai + c.aj * RP;
 #pragma unroll
     for (int it = 0; it < NIA; ++it) {
-      uint32_t i, j;
-      const bool ok = slot_a(c, it, i, j);
-      const uint32_t li = ok ? i + j * (uint32_t)p.RP : (uint32_t)p.lds_spare;
+      const bool ok = (int32_t)(it * SA) < c.limA;
+      const uint32_t a = ok ? li : (uint32_t)p.lds_spare;
 #pragma unroll
-      for (int v = 0; v < VA; ++v) data[li + v] = OP::prod(alpha, va[it][v]);
+      for (int v = 0; v < VA; ++v) data[a + v] = OP::prod(alpha, va[it][v]);
+      i += c.adi;
+      li += dli;
+      if (i >= c.rows) {
+        i -= c.rows;
+        li += wrap;
+      }
     }
   }
 
-  __device__ __forceinline__ uint32_t off_b(const RCParams& p, uint32_t i, uint32_t j) const {
-    return (uint32_t)(rowB[i] + (p.colB_dense ? (int32_t)j : colB[j])) * (uint32_t)sizeof(TB);
+  // The write walk: for each slot, its byte offset in B (out of range past
+  // the tile) and its LDS index; f(it, off, li).
+  template <bool DENSE, class F>
+  __device__ __forceinline__ void walk_b(const RCParams& p, const Tile& c, F&& f) const {
+    const uint32_t RP = (uint32_t)p.RP;
+    const uint32_t rmax = (uint32_t)p.R - 1;
+    const uint32_t dli = c.bdj * RP + c.bdi, wrap = 1u - c.cols * RP;
+    uint32_t i = c.bi, j = c.bj, li = c.bi + c.bj * RP;
+#pragma unroll
+    for (int it = 0; it < NIB; ++it) {
+      const bool ok = (int32_t)(it * SB) < c.limB;
+      const uint32_t ic = i < rmax ? i : rmax;
+      const uint32_t off = (uint32_t)rowB[ic] + (DENSE ? j * (uint32_t)sizeof(TB) : (uint32_t)colB[j]);
+      f(it, ok ? off : kBufOOB, ok ? li : 0u);
+      j += c.bdj;
+      i += c.bdi;
+      li += dli;
+      if (j >= c.cols) {
+        j -= c.cols;
+        ++i;
+        li += wrap;
+      }
+    }
   }
 
   // Old values of B for the beta epilogue.
-  __device__ __forceinline__ void load_b(const RCParams& p, const Coord& c, const TB* __restrict__ B,
+  __device__ __forceinline__ void load_b(const RCParams& p, const Tile& c, const TB* __restrict__ B,
                                          TB (&ob)[NIB][VB]) const {
     const BufRes r = make_res(B + c.bB);
-#pragma unroll
-    for (int it = 0; it < NIB; ++it) {
-      uint32_t i, j;
-      const bool ok = slot_b(c, it, i, j);
-      bload<TB, VB>(ob[it], r, ok ? off_b(p, i, j) : kBufOOB);
-    }
+    auto f = [&](int it, uint32_t off, uint32_t) { bload<TB, VB>(ob[it], r, off); };
+    if (p.colB_dense) walk_b<true>(p, c, f);
+    else walk_b<false>(p, c, f);
   }
 
-  __device__ __forceinline__ void store(const RCParams& p, const Coord& c, TB* __restrict__ B,
+  __device__ __forceinline__ void store(const RCParams& p, const Tile& c, TB* __restrict__ B,
                                         typename OP::BE beta, const TB (&ob)[NIB][VB]) const {
     const BufRes r = make_res(B + c.bB);
     const uint32_t RP = (uint32_t)p.RP;
-#pragma unroll
-    for (int it = 0; it < NIB; ++it) {
-      uint32_t i, j;
-      const bool ok = slot_b(c, it, i, j);
-      const uint32_t li = i + j * RP;
+    auto f = [&](int it, uint32_t off, uint32_t li) {
       TB o[VB];
 #pragma unroll
       for (int v = 0; v < VB; ++v) {
         if constexpr (BETA) o[v] = OP::outb(data[li + v * RP], beta, ob[it][v]);
         else o[v] = OP::out(data[li + v * RP]);
       }
-      bstore<TB, VB, NT>(r, ok ? off_b(p, i, j) : kBufOOB, o);
-    }
+      bstore<TB, VB, NT>(r, off, o);
+    };
+    if (p.colB_dense) walk_b<true>(p, c, f);
+    else walk_b<false>(p, c, f);
   }
 
   // Software pipeline over this work-group's tiles: the next tile's reads
@@ -193,21 +251,20 @@ struct RCWorker {
   // iteration, in issue order: old B of tile t (beta only), A of tile t+1,
   // B stores of tile t; staging tile t+1 then waits only for its own reads.
   // The last iteration prefetches an empty tile (all lanes out of range).
-  __device__ __forceinline__ void run(const RCParams& p, const TA* __restrict__ A, TB* __restrict__ B,
-                                      uint32_t first, uint32_t end, uint32_t step, typename OP::AL alpha,
-                                      typename OP::BE beta) {
+  __device__ __forceinline__ void run(const RCParams& p, KParams* kp, const TA* __restrict__ A,
+                                      TB* __restrict__ B, uint32_t first, uint32_t end, uint32_t step,
+                                      typename OP::AL alpha, typename OP::BE beta) {
     if (first >= end) return;
     TA va[NIA][VA];
     TB ob[NIB][VB];
-    Coord c = coord(p, first);
+    Tile c = tile(p, kp, first, false);
     load(p, c, A, va);
     for (uint32_t t = first; t < end; t += step) {
       stage(p, c, va, alpha);
       __syncthreads();
       if constexpr (BETA) load_b(p, c, B, ob);
       const uint32_t tn = t + step;
-      Coord cn = coord(p, tn < end ? tn : t);
-      if (tn >= end) cn.rows = 0;
+      const Tile cn = tile(p, kp, tn < end ? tn : t, tn >= end);
       load(p, cn, A, va);
       store(p, c, B, beta, ob);
       __syncthreads();
@@ -216,15 +273,21 @@ struct RCWorker {
   }
 };
 
+// The parameters are the kernel's first argument, so they sit at the start
+// of the kernel-argument segment.
 template <class OP, int VA, int VB, bool BETA, bool NT>
 __global__ void __launch_bounds__(kThreads) transpose_rc_kernel(const RCParams p, const typename OP::TA* __restrict__ A,
                                                                 typename OP::TB* __restrict__ B, double alpha,
                                                                 double beta) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
+  KParams* kp = (KParams*)__builtin_amdgcn_kernarg_segment_ptr();
   RCWorker<OP, VA, VB, BETA, NT> w;
   w.setup(p, lds_raw, threadIdx.x);
   const TileRange r = tile_range(p.ntiles, blockIdx.x, gridDim.x, p.grid_order);
-  w.run(p, A, B, r.first, r.end, r.step, (typename OP::AL)alpha, (typename OP::BE)beta);
+  // Work-group-uniform by construction; readfirstlane keeps them (and all
+  // tile coordinates derived from them) in scalar registers.
+  w.run(p, kp, A, B, uniform(r.first), uniform(r.end), uniform(r.step), (typename OP::AL)alpha,
+        (typename OP::BE)beta);
 }
 
 }  // namespace ttc
