@@ -15,6 +15,12 @@ bounded sample.  `--batch` adds the grouped-launch comparison (512 mixed ops,
 one persistent launch, hipGraph replay) in transposes/s.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-batch]
+  python bench.py --calibrate-copy     (a plain 200 MiB copy, for counter calibration)
+
+roofline.traffic comes from the committed counter summary
+profiles/c2_hbm_traffic.json (rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this
+benchmark, scaled by the copy calibration; see ttc_amd/profiling.py) when it
+lists the dominant kernel, else null.
 """
 import argparse
 import json
@@ -26,6 +32,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "c2_hbm_traffic.json")
+CALIB_BYTES = 200 << 20
 
 
 def log(*a):
@@ -229,6 +237,29 @@ def run_batch(torch, rt, wl, dev, stream, replays):
             "grouped_GBps": round(info["bytes"] / (t_group * 1e-3) / 1e9, 1)}
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            data = json.load(f)
+    except (OSError, ValueError):
+        return None
+    k = data.get("kernels", {}).get(kernel)
+    return k["traffic_bytes"] if k else None
+
+
+def calibrate_copy(torch):
+    """A plain device copy of CALIB_BYTES, three times (the reference stream
+    for the counter scale factors)."""
+    dev = torch.device("cuda:0")
+    x = torch.ones(CALIB_BYTES // 4, dtype=torch.float32, device=dev)
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": CALIB_BYTES, "copies": 3}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -239,11 +270,12 @@ def main():
     ap.add_argument("--batch", dest="batch", action="store_true", default=True)
     ap.add_argument("--no-batch", dest="batch", action="store_false")
     ap.add_argument("--batch-replays", type=int, default=200)
+    ap.add_argument("--calibrate-copy", action="store_true", help="run only the plain copy used to scale counters")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU only (every box has one MI355X); "
-              "--gpus %d is not supported — the path would shard as independent sub-transpositions "
-              "(see DESIGN.md)" % args.gpus, file=sys.stderr)
+        print("bench.py measures one GPU only; --gpus %d is not supported (a transposition shards as "
+              "independent sub-transpositions, one slab of B's outermost dimension per GPU, with no "
+              "exchange)" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -253,6 +285,9 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py: no GPU", file=sys.stderr)
         sys.exit(1)
+    if args.calibrate_copy:
+        calibrate_copy(torch)
+        return
     dev = torch.device("cuda:0")
     rt.lib()
     stream = torch.cuda.Stream(device=dev)
@@ -292,7 +327,7 @@ def main():
                                "(64 transposes per step)", "bytes_per_step": step_bytes,
                    "pct_of_hbm_peak": round(100.0 * value / HBM_PEAK_GBS, 2), "verified_sample": ok},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom),
                      "avg_launch_ms": round(gd["ms"] / gd["launches"], 5),
                      "bytes_per_launch": gd["bytes"] // gd["launches"]},
     }

@@ -3,7 +3,7 @@
 The oracle is a CPU restatement of HPAC/TTC's generated scalar reference
 loop nest (ttc/transpose.py:126-156, 394-473, 570-573 as configured by
 ttc/transposeGenerator.py:174-180); see ttc_oracle.cpp for the per-function
-citations.  Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline
+citations.  Only tests/, the package smoke check and bench.py's CPU-baseline
 leg may import this module.  The product (ttc_amd) never does.
 
 Arrays are numpy arrays holding the *flat memory* of column-major tensors

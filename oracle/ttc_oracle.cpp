@@ -2,7 +2,7 @@
 //
 // A CPU restatement of the scalar reference loop nest that HPAC/TTC generates
 // for  B[pi(i)] = alpha * A[i] + beta * B[pi(i)]  (column-major, index 0 is
-// stride-1).  Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline
+// stride-1).  Only tests/, the package smoke check and bench.py's CPU-baseline
 // leg may load this library, and only as the checker or the timed CPU
 // baseline; the product path (libttc_hip.so) never calls it.
 //

@@ -174,3 +174,18 @@ def to_device(arr, device):
 
 def from_device(t, like):
     return t.cpu().numpy().view(like.dtype).reshape(like.shape)
+
+
+# ---------------------------------------------------------------- fixtures
+GOLDEN_CODES = ["s", "d", "c", "z", "sd", "ds", "cz", "zc", "b", "h", "sb", "sh", "bs", "hs"]
+
+
+def load_golden(name):
+    """(meta, arrays) of tests/golden/<name>.npz (written by make_golden.py)."""
+    import json
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", name + ".npz")
+    with np.load(path, allow_pickle=False) as z:
+        arrays = {k: z[k] for k in z.files}
+    meta = json.loads(bytes(arrays.pop("index")).decode())
+    return meta, arrays

@@ -3,7 +3,7 @@
 Same queries and semantics as the reference (size row, permutation row,
 measurement row keyed by types, beta == 0 or not, thread count, hotA/hotB
 and architecture; best variant by bandwidth), written with bound parameters.
-The GPU variant rows carry this build's kernel configuration in extra
+The GPU variant rows carry ttc_amd's kernel configuration in extra
 columns (see ttc.createTables).
 """
 import json

@@ -232,7 +232,7 @@ def generateTransposition(ttcArgs):
     when nothing was measured).  Emits ./ttc_transpositions/<name>.hip/.h."""
     from . import runtime as rt
     if ttcArgs.architecture not in GPU_ARCHS:
-        _fail("architecture '%s': this build generates GPU (gfx950) transpositions only; use "
+        _fail("architecture '%s': ttc_amd generates GPU (gfx950) transpositions only; use "
               "--architecture=hip" % ttcArgs.architecture)
     if ttcArgs.architecture == "cuda" and not ttcArgs.silent:
         print(WARNING + "[TTC] WARNING: 'cuda' selected -> generating for HIP/gfx950 instead." + ENDC)
@@ -246,7 +246,7 @@ def generateTransposition(ttcArgs):
     if ttcArgs.maxNumImplementations == -1:
         ttcArgs.maxNumImplementations = 10000000
     if len(ttcArgs.vecLength) > 0 and any(v != 256 for v in ttcArgs.vecLength):
-        _fail("--threadsPerBlock: this build's kernels run 256 threads per work-group")
+        _fail("--threadsPerBlock: ttc_amd's kernels run 256 threads per work-group")
 
     # ---------------------------------------------------- fuse indices
     sizeOld, permOld = copy.deepcopy(ttcArgs.size), copy.deepcopy(ttcArgs.idxPerm)
@@ -425,7 +425,7 @@ def parse_args(argv):
                 opts["types"] = ttc_util.dataTypeFromString(val)
             except ValueError:
                 print("[TTC] ERROR: unknown precision. It needs to be one of s, d, c, z, sd, ds, cz, zc "
-                      "(or this build's b, h, sb, sh, bs, hs, sq, sr, bq, br, hq, hr, typeA->typeB).")
+                      "(or ttc_amd's b, h, sb, sh, bs, hs, sq, sr, bq, br, hq, hr, typeA->typeB).")
                 sys.exit(-1)
         elif key == "--numThreads":
             opts["numThreads"] = int(val)

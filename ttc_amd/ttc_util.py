@@ -3,7 +3,7 @@
 Same names and argument meaning as the reference so that code written
 against ``ttc.ttc_util`` (e.g. test-framework/ttc_test.py:124-145) reads the
 same.  Floating-point type strings follow the reference ("float", "double",
-"float complex", "double complex") plus the types this build adds
+"float complex", "double complex") plus the types ttc_amd adds
 ("bfloat16", "half", "fp8_e4m3", "fp8_e5m2").
 """
 import copy
@@ -28,7 +28,7 @@ FLOAT_TYPES = {
 
 
 def getVersion():
-    """ttc/ttc_util.py:10-11 (this build's own version)."""
+    """ttc/ttc_util.py:10-11 (ttc_amd's own version)."""
     return [0, 1, 0]
 
 
@@ -63,7 +63,7 @@ def listToString(l):
     return "(" + ",".join(str(x) for x in l) + ")"
 
 
-# --dataType codes: the reference's (ttc/ttc.py:1253-1280) and this build's.
+# --dataType codes: the reference's (ttc/ttc.py:1253-1280) and ttc_amd's.
 DATA_TYPES = {
     "s": ("float", "float"), "d": ("double", "double"), "c": ("float complex", "float complex"),
     "z": ("double complex", "double complex"), "sd": ("float", "double"), "ds": ("double", "float"),
