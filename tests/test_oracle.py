@@ -214,3 +214,28 @@ def test_fastdiv_math():
         for n in ns:
             q = (((n * m) >> 32) + n) >> s
             assert q == n // d, (n, d)
+
+
+def test_pinning_catches_planted_mistakes(oracle_lib):
+    """The numpy pin has teeth: plausible mistakes planted in the oracle's
+    inputs on the CPU (an inverted permutation, a dropped beta term, beta in
+    the wrong precision, A's padding ignored) each produce a mismatch."""
+    perm, size = [2, 0, 3, 1], [5, 4, 6, 3]
+    inv = [perm.index(k) for k in range(len(perm))]
+    A, B = make_case(S, S, perm, size, seed=11, dist="normal")
+    want = np_reference(S, S, A, B, 1.1, 0.7, perm, size)
+    assert same_bits(oracle_run(S, S, A, B, 1.1, 0.7, perm, size), want)
+    assert not same_bits(oracle_run(S, S, A, B, 1.1, 0.7, inv, size), want)            # inverse permutation
+    assert not same_bits(oracle_run(S, S, A, B, 1.1, 0.0, perm, size), want)           # beta term dropped
+    Ad, Bd = make_case(D, S, perm, size, seed=12, dist="normal")
+    want_ds = np_reference(D, S, Ad, Bd, 1.1, 0.1, perm, size)
+    # beta rounded in double instead of B's float changes some results of ds
+    wrong = np_reference(D, D, Ad, Bd.astype(np.float64), 1.1, 0.1, perm, size).astype(np.float32)
+    assert same_bits(oracle_run(D, S, Ad, Bd, 1.1, 0.1, perm, size), want_ds)
+    assert not same_bits(wrong, want_ds)
+    lda = [7, 5, 6, 3]
+    Ap, Bp = make_case(S, S, perm, size, lda, None, seed=13, dist="normal")
+    want_p = np_reference(S, S, Ap, Bp, 1.0, 1.0, perm, size, lda)
+    assert same_bits(oracle_run(S, S, Ap, Bp, 1.0, 1.0, perm, size, lda), want_p)
+    dense = oracle_run(S, S, Ap[:int(np.prod(size))], Bp, 1.0, 1.0, perm, size)          # padding ignored
+    assert not same_bits(dense, want_p)
