@@ -132,9 +132,9 @@ def _check_plan(i, perm, size):
 def _check_rc_groups(i):
     """A row x column tile: some split into rows = A's leading dimensions
     0..na-1 (never B's stride-1 dimension) and columns = B's leading
-    dimensions perm[0..nb-1], disjoint, covering every dimension with t > 1,
-    with a partial last tile only on a group's outermost dimension, whose
-    row and column counts are the plan's."""
+    dimensions perm[0..nb-1] followed by the other tiled dimensions,
+    disjoint, with a partial last tile only on a group's outermost
+    dimension, whose row and column counts are the plan's."""
     r, t, n, perm = i["rank"], i["tile"], i["size"], i["perm"]
     assert perm[0] != 0
     big = {k for k in range(r) if t[k] > 1}
@@ -143,8 +143,9 @@ def _check_rc_groups(i):
         ga = list(range(na))
         for nb in range(1, r + 1):
             gb = list(perm[:nb])
-            if set(ga) & set(gb) or not big <= set(ga) | set(gb):
+            if set(ga) & set(gb):
                 continue
+            gb = gb + sorted(big - set(ga) - set(gb))
             if any(n[k] % t[k] for k in ga[:-1] + gb[:-1]):
                 continue
             rows, cols = int(np.prod([t[k] for k in ga])), int(np.prod([t[k] for k in gb]))

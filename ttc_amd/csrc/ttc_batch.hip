@@ -3,14 +3,16 @@
 // replayed through a captured hipGraph.  The reference has no counterpart
 // (one transposition per host call, ttc/CUDAtranspose.py:395-429).
 //
-// Work table: one record per op (its tile parameters, pointers, scalars),
-// ops grouped by kernel kind (type pair, vector widths, beta read or not),
+// Work table: one record per op (its tile parameters, pointers, scalars) and
+// one record per tile (origin offsets, partial-tile limits), ops grouped by
+// kernel kind (kernel family, type pair, vector widths, beta read or not),
 // each group's tiles numbered consecutively, and a split of each group's tile
 // range into one contiguous slice per work-group, balanced by bytes moved.
-// One persistent launch per kind present walks that kind's slices; a
-// work-group whose slice crosses into the next op re-derives its per-thread
-// tile offsets for that op.  (One kernel for all kinds would carry the
-// register needs of its most demanding kind into every op: measured on
+// One persistent launch per kind present walks that kind's slices.  Ops whose
+// tiles have the row x column shape run the row x column kernel (a
+// work-group rebuilds its LDS offset tables when its slice crosses into the
+// next op); the others run box tiles.  (One kernel for all kinds would carry
+// the register needs of its most demanding kind into every op: measured on
 // gfx950 it either ran at one wave per SIMD or spilled.)
 #include <hip/hip_runtime.h>
 
@@ -57,10 +59,24 @@ static void fill_bases(const TileParams& p, TileBase* out) {
   }
 }
 
-// Kinds handled by the grouped kernel: same-type pairs, vector width 1 or the
-// widest on each side, beta read or not (non-temporal stores when not).
-// kind = ((pair * 2 + wide_a) * 2 + wide_b) * 2 + beta_is_zero
+// Kinds handled by the grouped kernels: same-type pairs, vector width 1 or
+// the widest on each side, beta read or not, box or row x column tiles.
+// kind = (((family * kBPairs + pair) * 2 + wide_a) * 2 + wide_b) * 2 + beta_is_zero
 enum BatchPair : int { kBS = 0, kBD = 1, kBBF16 = 2, kBF16 = 3, kBC = 4, kBZ = 5, kBPairs = 6 };
+constexpr int kKindsPerFamily = kBPairs * 8;
+
+// Op record of a row x column group.
+struct BatchRCOpDev {
+  RCParams p;
+  const void* A;
+  void* B;
+  double alpha, beta;
+  uint32_t tile_begin;    // first tile of this op in its group
+  uint32_t ntiles;
+  const RCTile* tiles;    // this op's tile records
+  const int32_t* tables;  // its offset tables: rowA[R], rowB[R], colA[C], colB[C]
+};
+using KRCOp = const __attribute__((address_space(4))) BatchRCOpDev;
 
 __device__ __forceinline__ int32_t uni(int32_t x) { return __builtin_amdgcn_readfirstlane(x); }
 __device__ __forceinline__ uint32_t uni(uint32_t x) { return (uint32_t)__builtin_amdgcn_readfirstlane((int32_t)x); }
@@ -140,7 +156,72 @@ __global__ void __launch_bounds__(kThreads) batch_group_kernel(const BatchOpDev*
   }
 }
 
-// The kinds: kind = ((pair * 2 + wide_a) * 2 + wide_b) * 2 + beta_is_zero.
+// The scalar shape parameters the row x column walks read (scalar loads
+// through the constant address space).
+__device__ __forceinline__ void scalar_rc(KRCOp& o, RCParams& p) {
+  p.R = o.p.R;
+  p.C = o.p.C;
+  p.Rl = o.p.Rl;
+  p.Cl = o.p.Cl;
+  p.dR.d = o.p.dR.d; p.dR.m = o.p.dR.m; p.dR.s = o.p.dR.s;
+  p.dRl.d = o.p.dRl.d; p.dRl.m = o.p.dRl.m; p.dRl.s = o.p.dRl.s;
+  p.dC.d = o.p.dC.d; p.dC.m = o.p.dC.m; p.dC.s = o.p.dC.s;
+  p.dCl.d = o.p.dCl.d; p.dCl.m = o.p.dCl.m; p.dCl.s = o.p.dCl.s;
+  p.RP = o.p.RP;
+  p.rowA_dense = o.p.rowA_dense;
+  p.colB_dense = o.p.colB_dense;
+  p.lds_data_elems = o.p.lds_data_elems;
+  p.lds_spare = o.p.lds_spare;
+}
+
+template <class OP, int VA, int VB, bool BETA>
+__global__ void __launch_bounds__(kRcThreads) batch_rc_group_kernel(const BatchRCOpDev* __restrict__ ops, int nops,
+                                                                    const uint32_t* __restrict__ wg_begin) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  const uint32_t t0 = wg_begin[blockIdx.x], t1 = wg_begin[blockIdx.x + 1];
+  if (t0 >= t1) return;
+  int lo = 0, hi = nops - 1;  // last op with tile_begin <= t0
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (ops[mid].tile_begin <= t0) lo = mid;
+    else hi = mid - 1;
+  }
+  KRCOp* kops = (KRCOp*)ops;
+  RCWorker<OP, VA, VB, BETA, false> w;
+  uint32_t t = t0;
+  for (int op = lo; op < nops && t < t1; ++op) {
+    KRCOp& o = kops[op];
+    const uint32_t end = o.tile_begin + o.ntiles;
+    const uint32_t stop = end < t1 ? end : t1;
+    if (t < stop) {
+      RCParams up;
+      scalar_rc(o, up);
+      // ends with a barrier; the previous op's last tile ended with one
+      w.setup_tables(up, o.tables, lds_raw, threadIdx.x);
+      rc_run_table(w, up, (KTile*)o.tiles, t - o.tile_begin, stop - o.tile_begin,
+                   static_cast<const typename OP::TA*>(o.A), static_cast<typename OP::TB*>(o.B),
+                   (typename OP::AL)o.alpha, (typename OP::BE)o.beta);
+      t = stop;
+    }
+  }
+}
+
+// The kinds of one family: ((pair * 2 + wide_a) * 2 + wide_b) * 2 + beta_is_zero.
+template <class OP, int WA, int WB>
+static void add_rc_pair(const void** tab, int pair) {
+  const void* k[8] = {
+      reinterpret_cast<const void*>(&batch_rc_group_kernel<OP, 1, 1, true>),
+      reinterpret_cast<const void*>(&batch_rc_group_kernel<OP, 1, 1, false>),
+      reinterpret_cast<const void*>(&batch_rc_group_kernel<OP, 1, WB, true>),
+      reinterpret_cast<const void*>(&batch_rc_group_kernel<OP, 1, WB, false>),
+      reinterpret_cast<const void*>(&batch_rc_group_kernel<OP, WA, 1, true>),
+      reinterpret_cast<const void*>(&batch_rc_group_kernel<OP, WA, 1, false>),
+      reinterpret_cast<const void*>(&batch_rc_group_kernel<OP, WA, WB, true>),
+      reinterpret_cast<const void*>(&batch_rc_group_kernel<OP, WA, WB, false>),
+  };
+  for (int i = 0; i < 8; ++i) tab[kKindsPerFamily + pair * 8 + i] = k[i];
+}
+
 template <class OP, int WA, int WB>
 static void add_pair(const void** tab, int pair) {
   const void* k[8] = {
@@ -157,7 +238,7 @@ static void add_pair(const void** tab, int pair) {
 }
 
 struct GroupKernels {
-  const void* k[kBPairs * 8];
+  const void* k[2 * kKindsPerFamily];
   GroupKernels() {
     add_pair<OpF32<float, float>, 4, 4>(k, kBS);
     add_pair<OpDD, 2, 2>(k, kBD);
@@ -165,8 +246,15 @@ struct GroupKernels {
     add_pair<OpF32<f16s, f16s>, 8, 8>(k, kBF16);
     add_pair<OpCC, 2, 2>(k, kBC);
     add_pair<OpZZ, 1, 1>(k, kBZ);
+    add_rc_pair<OpF32<float, float>, 4, 4>(k, kBS);
+    add_rc_pair<OpDD, 2, 2>(k, kBD);
+    add_rc_pair<OpF32<bf16s, bf16s>, 8, 8>(k, kBBF16);
+    add_rc_pair<OpF32<f16s, f16s>, 8, 8>(k, kBF16);
+    add_rc_pair<OpCC, 2, 2>(k, kBC);
+    add_rc_pair<OpZZ, 1, 1>(k, kBZ);
   }
 };
+static bool kind_is_rc(int kind) { return kind >= kKindsPerFamily; }
 static const void* group_kernel(int kind) {
   static const GroupKernels tab;  // thread-safe initialisation
   return tab.k[kind];
@@ -192,7 +280,7 @@ using namespace ttc;
 // One launch of the grouped execution: the ops of one kind.
 struct BatchGroup {
   int kind;
-  int op0, nops;   // slice of the device op table
+  int op0, nops;   // slice of the device op table of the kind's family
   int wg0, nwg;    // slice of the device work-group table (nwg + 1 entries)
   int lds_bytes;
 };
@@ -202,9 +290,12 @@ struct ttc_batch_s {
   std::vector<ttc_plan_t> plans;  // per-op plans (also used by the per-op loop)
   std::vector<ttc_batch_op_t> ops;
   std::vector<BatchGroup> groups;
-  BatchOpDev* d_ops;
+  BatchOpDev* d_ops;       // box ops
+  BatchRCOpDev* d_rc_ops;  // row x column ops
   uint32_t* d_wg;
   TileBase* d_bases;
+  RCTile* d_rc_tiles;
+  int32_t* d_rc_tables;
   int nwg_total;
   int64_t tiles, bytes;
   hipGraph_t graph[2];
@@ -215,8 +306,11 @@ static void batch_free(ttc_batch_s* b) {
   for (ttc_plan_t p : b->plans)
     if (p) ttc_plan_destroy(p);
   if (b->d_ops) (void)hipFree(b->d_ops);
+  if (b->d_rc_ops) (void)hipFree(b->d_rc_ops);
   if (b->d_wg) (void)hipFree(b->d_wg);
   if (b->d_bases) (void)hipFree(b->d_bases);
+  if (b->d_rc_tiles) (void)hipFree(b->d_rc_tiles);
+  if (b->d_rc_tables) (void)hipFree(b->d_rc_tables);
   for (int i = 0; i < 2; ++i) {
     if (b->exec[i]) (void)hipGraphExecDestroy(b->exec[i]);
     if (b->graph[i]) (void)hipGraphDestroy(b->graph[i]);
@@ -224,19 +318,76 @@ static void batch_free(ttc_batch_s* b) {
   delete b;
 }
 
+// Host version of the row x column grid walk (the kernel's tile()): one
+// record per tile, tiles numbered over the grid positions in plan order.
+static void fill_rc_tiles(const RCParams& p, RCTile* out) {
+  for (uint32_t t = 0; t < p.ntiles; ++t) {
+    uint32_t rem = t;
+    RCTile r{0, 0, 0, 0};
+    for (int g = 0; g < p.grank; ++g) {
+      const uint32_t n = p.gdiv[g].d;
+      const uint32_t x = rem % n;
+      rem /= n;
+      r.a += (int64_t)x * p.tsa[g];
+      r.b += (int64_t)x * p.tsb[g];
+      const bool partial = (uint32_t)p.n[g] - x * (uint32_t)p.t[g] < (uint32_t)p.t[g];
+      if (g == p.splitA && partial) r.flags |= 1u;
+      if (g == p.splitB && partial) r.flags |= 2u;
+    }
+    out[t] = r;
+  }
+}
+
+// The offset tables of a row x column op, as the kernel's setup() builds
+// them: rowA[R], rowB[R], colA[C], colB[C] (byte offsets).
+static void rc_tables(const RCParams& p, int32_t* out) {
+  for (int i = 0; i < p.R; ++i) {
+    uint32_t l = (uint32_t)i;
+    int32_t oa = 0, ob = 0;
+    for (int m = 0; m < p.na; ++m) {
+      const uint32_t d = l % (uint32_t)p.ta[m];
+      l /= (uint32_t)p.ta[m];
+      oa += (int32_t)d * p.saA[m];
+      ob += (int32_t)d * p.sbA[m];
+    }
+    out[i] = oa;
+    out[p.R + i] = ob;
+  }
+  for (int j = 0; j < p.C; ++j) {
+    uint32_t l = (uint32_t)j;
+    int32_t oa = 0, ob = 0;
+    for (int m = 0; m < p.nb; ++m) {
+      const uint32_t d = l % (uint32_t)p.tb[m];
+      l /= (uint32_t)p.tb[m];
+      oa += (int32_t)d * p.saB[m];
+      ob += (int32_t)d * p.sbB[m];
+    }
+    out[2 * p.R + j] = oa;
+    out[2 * p.R + p.C + j] = ob;
+  }
+}
+
+// One op of the work table while it is built.
+struct OpRec {
+  int kind;
+  uint32_t ntiles, tile_begin;
+  double weight;  // bytes moved per tile
+  int src;        // index into the caller's op list
+};
+
 // Work-group slices of one group's tile range, balanced by bytes: returns
-// nwg + 1 boundaries (global tile indices of the group).
-static std::vector<uint32_t> split_by_bytes(const std::vector<BatchOpDev>& h, const std::vector<double>& weight,
-                                            int op0, int nops, uint32_t tiles, int nwg) {
+// nwg + 1 boundaries (tile indices of the group).
+static std::vector<uint32_t> split_by_bytes(const std::vector<OpRec>& h, int op0, int nops, uint32_t tiles,
+                                            int nwg) {
   std::vector<uint32_t> wgb(nwg + 1, tiles);
   double total = 0;
-  for (int i = op0; i < op0 + nops; ++i) total += weight[i] * h[i].ntiles;
+  for (int i = op0; i < op0 + nops; ++i) total += h[i].weight * h[i].ntiles;
   const double per = total / nwg;
   int w = 1;
   double acc = 0;
   wgb[0] = 0;
   for (int i = op0; i < op0 + nops && w < nwg; ++i) {
-    const double wt = weight[i];
+    const double wt = h[i].weight;
     const uint32_t n = h[i].ntiles;
     if (n == 0) continue;
     while (w < nwg && acc + wt * n >= per * w) {
@@ -261,8 +412,7 @@ extern "C" ttc_status_t ttc_batch_create(ttc_batch_t* out, int nops, const ttc_b
   b->nops = nops;
   b->ops.assign(ops, ops + nops);
   b->plans.assign(nops, nullptr);
-  std::vector<BatchOpDev> rec(nops);
-  std::vector<int> kinds(nops);
+  std::vector<OpRec> rec(nops);
   b->bytes = 0;
   for (int i = 0; i < nops; ++i) {
     const ttc_batch_op_t& op = ops[i];
@@ -280,10 +430,8 @@ extern "C" ttc_status_t ttc_batch_create(ttc_batch_t* out, int nops, const ttc_b
       any_lda |= op.lda[k] != 0;
       any_ldb |= op.ldb[k] != 0;
     }
-    // vector widths: 1 or the widest of the type (the kinds instantiated);
-    // the grouped kernels run box tiles
+    // vector widths: 1 or the widest of the type (the kinds instantiated)
     ttc_config_t cfg = op.cfg;
-    cfg.family = kFamilyBox;
     const int wide = std::max(1, 16 / dtype_bytes(op.type_a));
     for (int pass = 0; pass < 2; ++pass) {
       ttc_plan_t pl = nullptr;
@@ -319,43 +467,77 @@ extern "C" ttc_status_t ttc_batch_create(ttc_batch_t* out, int nops, const ttc_b
       batch_free(b);
       return TTC_ERROR_INVALID_VALUE;
     }
-    BatchOpDev& d = rec[i];
-    std::memset(&d, 0, sizeof(d));
-    d.p = use->p;
-    d.A = op.A;
-    d.B = op.B;
-    d.alpha = op.alpha;
-    d.beta = op.beta;
     const int wa = use->va > 1 ? 1 : 0, wb = use->vb > 1 ? 1 : 0;
-    d.kind = ((pair * 2 + wa) * 2 + wb) * 2 + (pl->beta_is_zero ? 1 : 0);
-    d.ntiles = pl->g.elements > 0 ? use->p.ntiles : 0;
-    kinds[i] = d.kind;
+    const int fam = use->family == kFamilyRC ? 1 : 0;
+    OpRec& r = rec[i];
+    r.kind = (((fam * kBPairs + pair) * 2 + wa) * 2 + wb) * 2 + (pl->beta_is_zero ? 1 : 0);
+    r.ntiles = pl->g.elements > 0 ? use->p.ntiles : 0;
+    r.src = i;
     const int64_t sa = dtype_bytes(op.type_a), sb = dtype_bytes(op.type_b);
+    r.weight = r.ntiles ? (double)pl->g.elements * (double)(sa + sb * (pl->beta_is_zero ? 1 : 2)) / r.ntiles : 0.0;
     b->bytes += pl->g.elements * (sa + sb * (pl->beta_is_zero ? 1 : 2));
   }
   // order the table by kind (stable: ops keep their order inside a kind)
-  std::vector<int> order(nops);
-  for (int i = 0; i < nops; ++i) order[i] = i;
-  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return kinds[x] < kinds[y]; });
-  std::vector<BatchOpDev> h(nops);
-  std::vector<double> weight(nops);
-  for (int j = 0; j < nops; ++j) h[j] = rec[order[j]];
+  std::stable_sort(rec.begin(), rec.end(), [](const OpRec& x, const OpRec& y) { return x.kind < y.kind; });
   int dev = 0;
   (void)hipGetDevice(&dev);
   const int cus = device_cus(dev);
   std::vector<uint32_t> wg_all;
+  std::vector<BatchOpDev> box;
+  std::vector<BatchRCOpDev> rcs;
+  std::vector<TileBase> bases;
+  std::vector<RCTile> rct;
+  std::vector<int32_t> rtab;
+  std::vector<size_t> rec_slot(nops);  // index of each op's first tile record in its family's table
+  std::vector<size_t> tab_slot(nops);  // index of a row x column op's offset tables
   b->tiles = 0;
   for (int j = 0; j < nops;) {
     int e = j;
     uint64_t tiles = 0;
     int lds = 16;
-    while (e < nops && h[e].kind == h[j].kind) {
-      h[e].tile_begin = (uint32_t)tiles;
-      tiles += h[e].ntiles;
-      lds = std::max(lds, h[e].p.lds_elems * lds_elem_bytes(ops[order[e]].type_a, ops[order[e]].type_b));
-      const double bytes = (double)(dtype_bytes(ops[order[e]].type_a) +
-                                    dtype_bytes(ops[order[e]].type_b) * ((h[e].kind & 1) ? 1 : 2));
-      weight[e] = h[e].ntiles ? bytes * (double)h[e].p.V : 0.0;
+    const bool rc = kind_is_rc(rec[j].kind);
+    const size_t op0 = rc ? rcs.size() : box.size();
+    while (e < nops && rec[e].kind == rec[j].kind) {
+      OpRec& r = rec[e];
+      r.tile_begin = (uint32_t)tiles;
+      tiles += r.ntiles;
+      const ttc_batch_op_t& op = ops[r.src];
+      const ttc_plan_s* pl = b->plans[r.src];
+      const ttc_plan_s* use = pl->g.elements > 0 ? variant_for(pl, op.A, op.B) : pl;
+      lds = std::max(lds, use->lds_bytes);
+      if (rc) {
+        BatchRCOpDev d;
+        std::memset(&d, 0, sizeof(d));
+        d.p = use->rc;
+        d.A = op.A;
+        d.B = op.B;
+        d.alpha = op.alpha;
+        d.beta = op.beta;
+        d.tile_begin = r.tile_begin;
+        d.ntiles = r.ntiles;
+        rec_slot[e] = rct.size();  // first tile record of this op
+        rct.resize(rct.size() + r.ntiles);
+        if (r.ntiles) fill_rc_tiles(use->rc, rct.data() + rec_slot[e]);
+        tab_slot[e] = rtab.size();
+        rtab.resize(rtab.size() + 2 * (size_t)use->rc.R + 2 * (size_t)use->rc.C);
+        if (r.ntiles) rc_tables(use->rc, rtab.data() + tab_slot[e]);
+        rcs.push_back(d);
+      } else {
+        BatchOpDev d;
+        std::memset(&d, 0, sizeof(d));
+        d.p = use->p;
+        d.A = op.A;
+        d.B = op.B;
+        d.alpha = op.alpha;
+        d.beta = op.beta;
+        d.kind = r.kind;
+        d.tile_begin = r.tile_begin;
+        d.ntiles = r.ntiles;
+        rec_slot[e] = bases.size();
+        bases.resize(bases.size() + r.ntiles);
+        if (r.ntiles) fill_bases(use->p, bases.data() + rec_slot[e]);
+        box.push_back(d);
+      }
       ++e;
     }
     if (tiles >= (1ull << 31)) {
@@ -365,17 +547,18 @@ extern "C" ttc_status_t ttc_batch_create(ttc_batch_t* out, int nops, const ttc_b
     lds = (lds + 15) / 16 * 16;
     if (tiles > 0) {
       BatchGroup grp;
-      grp.kind = h[j].kind;
-      grp.op0 = j;
+      grp.kind = rec[j].kind;
+      grp.op0 = (int)op0;
       grp.nops = e - j;
       grp.lds_bytes = lds;
       const void* fn = group_kernel(grp.kind);
-      const int occ = occupancy(fn, lds);
-      int wg = wg_per_cu > 0 ? wg_per_cu : std::min(occ, 4);
+      const int threads = rc ? kRcThreads : kThreads;
+      const int occ = occupancy(fn, lds, threads);
+      int wg = wg_per_cu > 0 ? wg_per_cu : std::min(occ, 4 * kThreads / threads);
       if (wg < 1) wg = 1;
       grp.nwg = (int)std::min<uint64_t>((uint64_t)cus * wg, tiles);
       grp.wg0 = (int)wg_all.size();
-      std::vector<uint32_t> wgb = split_by_bytes(h, weight, j, e - j, (uint32_t)tiles, grp.nwg);
+      std::vector<uint32_t> wgb = split_by_bytes(rec, j, e - j, (uint32_t)tiles, grp.nwg);
       wg_all.insert(wg_all.end(), wgb.begin(), wgb.end());
       b->groups.push_back(grp);
       b->tiles += (int64_t)tiles;
@@ -384,28 +567,37 @@ extern "C" ttc_status_t ttc_batch_create(ttc_batch_t* out, int nops, const ttc_b
   }
   b->nwg_total = 0;
   for (const BatchGroup& g : b->groups) b->nwg_total += g.nwg;
-  // per-tile records of every op, in table order
-  std::vector<TileBase> bases((size_t)std::max<int64_t>(b->tiles, 1));
-  std::vector<size_t> base_off(nops);
-  {
-    size_t off = 0;
-    for (int j = 0; j < nops; ++j) {
-      base_off[j] = off;
-      if (h[j].ntiles) fill_bases(h[j].p, bases.data() + off);
-      off += h[j].ntiles;
-    }
-  }
-  if (hipMalloc(&b->d_ops, sizeof(BatchOpDev) * nops) != hipSuccess ||
+  if (hipMalloc(&b->d_ops, sizeof(BatchOpDev) * std::max<size_t>(box.size(), 1)) != hipSuccess ||
+      hipMalloc(&b->d_rc_ops, sizeof(BatchRCOpDev) * std::max<size_t>(rcs.size(), 1)) != hipSuccess ||
       hipMalloc(&b->d_wg, sizeof(uint32_t) * std::max<size_t>(wg_all.size(), 1)) != hipSuccess ||
-      hipMalloc(&b->d_bases, sizeof(TileBase) * bases.size()) != hipSuccess) {
+      hipMalloc(&b->d_bases, sizeof(TileBase) * std::max<size_t>(bases.size(), 1)) != hipSuccess ||
+      hipMalloc(&b->d_rc_tiles, sizeof(RCTile) * std::max<size_t>(rct.size(), 1)) != hipSuccess ||
+      hipMalloc(&b->d_rc_tables, sizeof(int32_t) * std::max<size_t>(rtab.size(), 1)) != hipSuccess) {
     batch_free(b);
     return TTC_ERROR_ALLOC;
   }
-  for (int j = 0; j < nops; ++j) h[j].bases = b->d_bases + base_off[j];
-  if (hipMemcpy(b->d_ops, h.data(), sizeof(BatchOpDev) * nops, hipMemcpyHostToDevice) != hipSuccess ||
-      (!wg_all.empty() && hipMemcpy(b->d_wg, wg_all.data(), sizeof(uint32_t) * wg_all.size(),
-                                    hipMemcpyHostToDevice) != hipSuccess) ||
-      hipMemcpy(b->d_bases, bases.data(), sizeof(TileBase) * bases.size(), hipMemcpyHostToDevice) != hipSuccess) {
+  // point each op record at its tile records
+  {
+    size_t ib = 0, ir = 0;
+    for (int j = 0; j < nops; ++j) {
+      if (kind_is_rc(rec[j].kind)) {
+        rcs[ir].tiles = b->d_rc_tiles + rec_slot[j];
+        rcs[ir].tables = b->d_rc_tables + tab_slot[j];
+        ++ir;
+      } else {
+        box[ib++].bases = b->d_bases + rec_slot[j];
+      }
+    }
+  }
+  auto up = [](void* dst, const void* src, size_t n) {
+    return n == 0 || hipMemcpy(dst, src, n, hipMemcpyHostToDevice) == hipSuccess;
+  };
+  if (!up(b->d_ops, box.data(), sizeof(BatchOpDev) * box.size()) ||
+      !up(b->d_rc_ops, rcs.data(), sizeof(BatchRCOpDev) * rcs.size()) ||
+      !up(b->d_wg, wg_all.data(), sizeof(uint32_t) * wg_all.size()) ||
+      !up(b->d_bases, bases.data(), sizeof(TileBase) * bases.size()) ||
+      !up(b->d_rc_tiles, rct.data(), sizeof(RCTile) * rct.size()) ||
+      !up(b->d_rc_tables, rtab.data(), sizeof(int32_t) * rtab.size())) {
     batch_free(b);
     return TTC_ERROR_HIP;
   }
@@ -416,12 +608,13 @@ extern "C" ttc_status_t ttc_batch_create(ttc_batch_t* out, int nops, const ttc_b
 extern "C" ttc_status_t ttc_batch_execute(ttc_batch_t b, ttc_stream_t stream) {
   if (!b) return TTC_ERROR_INVALID_VALUE;
   for (const BatchGroup& g : b->groups) {
-    const BatchOpDev* ops = b->d_ops + g.op0;
+    const bool rc = kind_is_rc(g.kind);
+    const void* ops = rc ? static_cast<const void*>(b->d_rc_ops + g.op0) : static_cast<const void*>(b->d_ops + g.op0);
     int nops = g.nops;
     const uint32_t* wg = b->d_wg + g.wg0;
     void* args[] = {&ops, &nops, &wg};
-    hipError_t e = hipLaunchKernel(group_kernel(g.kind), dim3(g.nwg), dim3(kThreads), args, (size_t)g.lds_bytes,
-                                   static_cast<hipStream_t>(stream));
+    hipError_t e = hipLaunchKernel(group_kernel(g.kind), dim3(g.nwg), dim3(rc ? kRcThreads : kThreads), args,
+                                   (size_t)g.lds_bytes, static_cast<hipStream_t>(stream));
     if (e != hipSuccess) return TTC_ERROR_HIP;
   }
   return TTC_SUCCESS;
@@ -442,13 +635,40 @@ extern "C" ttc_status_t ttc_batch_graph_launch(ttc_batch_t b, ttc_stream_t strea
   const int w = per_op_loop ? 1 : 0;
   hipStream_t s = static_cast<hipStream_t>(stream);
   if (!b->exec[w]) {
-    if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) return TTC_ERROR_HIP;
-    ttc_status_t st = w ? ttc_batch_execute_loop(b, stream) : ttc_batch_execute(b, stream);
     hipGraph_t g = nullptr;
-    hipError_t e = hipStreamEndCapture(s, &g);
-    if (st != TTC_SUCCESS || e != hipSuccess || !g) {
-      if (g) (void)hipGraphDestroy(g);
-      return st != TTC_SUCCESS ? st : TTC_ERROR_HIP;
+    if (w) {
+      // the per-op loop as a caller would capture it: one launch after another
+      if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) return TTC_ERROR_HIP;
+      ttc_status_t st = ttc_batch_execute_loop(b, stream);
+      hipError_t e = hipStreamEndCapture(s, &g);
+      if (st != TTC_SUCCESS || e != hipSuccess || !g) {
+        if (g) (void)hipGraphDestroy(g);
+        return st != TTC_SUCCESS ? st : TTC_ERROR_HIP;
+      }
+    } else {
+      // the grouped launch: one kernel node per kind, no edges between them
+      // (they touch disjoint ops), so the graph may run kinds side by side
+      if (hipGraphCreate(&g, 0) != hipSuccess) return TTC_ERROR_HIP;
+      for (const BatchGroup& gr : b->groups) {
+        const bool rc = kind_is_rc(gr.kind);
+        const void* ops = rc ? static_cast<const void*>(b->d_rc_ops + gr.op0)
+                             : static_cast<const void*>(b->d_ops + gr.op0);
+        int nops = gr.nops;
+        const uint32_t* wg = b->d_wg + gr.wg0;
+        void* args[] = {&ops, &nops, &wg};
+        hipKernelNodeParams kp;
+        std::memset(&kp, 0, sizeof(kp));
+        kp.func = const_cast<void*>(group_kernel(gr.kind));
+        kp.gridDim = dim3(gr.nwg);
+        kp.blockDim = dim3(rc ? kRcThreads : kThreads);
+        kp.sharedMemBytes = (unsigned)gr.lds_bytes;
+        kp.kernelParams = args;
+        hipGraphNode_t node;
+        if (hipGraphAddKernelNode(&node, g, nullptr, 0, &kp) != hipSuccess) {
+          (void)hipGraphDestroy(g);
+          return TTC_ERROR_HIP;
+        }
+      }
     }
     hipGraphExec_t x = nullptr;
     if (hipGraphInstantiate(&x, g, nullptr, nullptr, 0) != hipSuccess) {

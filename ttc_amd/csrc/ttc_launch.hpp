@@ -60,7 +60,7 @@ hipError_t launch_rc(const RCParams& p, const void* A, void* B, double alpha, do
   RCParams pp = p;
   void* args[] = {&pp, &a, &b, &alpha, &beta};
   return hipLaunchKernel(reinterpret_cast<const void*>(&transpose_rc_kernel<OP, VA, VB, BETA, NT>), dim3(grid),
-                         dim3(kThreads), args, lds, s);
+                         dim3(kRcThreads), args, lds, s);
 }
 
 template <class OP, int VA, int VB, int MODE>

@@ -14,6 +14,7 @@ namespace ttc {
 
 constexpr int kMaxDim = 8;     // fused rank limit
 constexpr int kThreads = 256;  // work-group size (4 wave64)
+constexpr int kRcThreads = 256;  // work-group size of the row x column kernels
 
 // Vectors a thread owns per tile side for vectors of `vec` elements of
 // `bytes` bytes: as many as fit 64 bytes of data (16 VGPRs), between 4 and
@@ -104,7 +105,16 @@ struct RCParams {
   int32_t lds_spare;       // RP * C: where lanes past a partial tile stage their (unused) values
 };
 
-// Elements per thread per tile side: as many as fit 128 bytes, at most 32.
+// One row x column tile of a grouped launch: element offsets of its origin
+// in A and B; flags bit 0: last (partial) tile along the row group, bit 1:
+// along the column group.
+struct RCTile {
+  int64_t a, b;
+  uint32_t flags, pad_;
+};
+
+// Elements per thread per tile side of the row x column kernels: as many as
+// fit 128 bytes, at most 32.
 constexpr int rc_elems(int bytes) { return 128 / bytes > 32 ? 32 : 128 / bytes; }
 
 }  // namespace ttc

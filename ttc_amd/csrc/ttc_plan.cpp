@@ -468,12 +468,13 @@ ttc_status_t build_params(const Geometry& g, const int* tile, int ta, int tb, in
 }
 
 // ------------------------------------------------------ row x column family
-// LDS per work-group (tile data plus offset tables): four work-groups per CU.
-static constexpr int kRcLdsBudget = 40960;
+// LDS per work-group (tile data plus offset tables): four 256-thread
+// work-groups per CU (160 KiB of LDS).
+static constexpr int kRcLdsBudget = 40960 * (kRcThreads / 256);
 
 int max_rc_volume(int ta, int tb) {
-  const int v = kThreads * std::min(rc_elems(dtype_bytes(ta)), rc_elems(dtype_bytes(tb)));
-  return std::min(v, 32768 / lds_elem_bytes(ta, tb));
+  const int v = kRcThreads * std::min(rc_elems(dtype_bytes(ta)), rc_elems(dtype_bytes(tb)));
+  return std::min(v, 32768 * (kRcThreads / 256) / lds_elem_bytes(ta, tb));
 }
 
 // Tile data (odd pitch RP: the write phase reads LDS with stride RP) plus a
@@ -487,10 +488,10 @@ static long long rc_lds(long long R, long long C, int lds_elem) {
 
 // Rows: A's leading dimensions 0, 1, ... up to B's stride-1 dimension;
 // columns: B's leading dimensions perm[0], perm[1], ... up to a row
-// dimension.  A group stops after a dimension whose extent the tile does not
-// divide (only a group's outermost dimension may have a partial last tile)
-// and before a dimension the tile does not cover (t == 1).  Every dimension
-// with t > 1 must land in a group.
+// dimension, then any other tiled dimension (in A order).  A group's walk
+// stops after a dimension whose extent the tile does not divide and before a
+// dimension the tile does not cover (t == 1); only a group's outermost
+// dimension may have a partial last tile.
 bool rc_groups(const Geometry& g, const int* tile, int& na, int* ga, int& nb, int* gb) {
   const int r = g.rank;
   const int p0 = g.perm[0];
@@ -512,9 +513,38 @@ bool rc_groups(const Geometry& g, const int* tile, int& na, int* ga, int& nb, in
     gb[nb++] = k;
     in_b[k] = true;
   }
-  for (int k = 0; k < r; ++k)
-    if (tile[k] > 1 && !in_a[k] && !in_b[k]) return false;
+  for (int k = 0; k < r; ++k) {
+    if (tile[k] <= 1 || in_a[k] || in_b[k]) continue;
+    gb[nb++] = k;  // an extra column dimension
+    in_b[k] = true;
+  }
+  for (int m = 0; m + 1 < nb; ++m)
+    if (g.size[gb[m]] % tile[gb[m]] != 0) return false;
   return true;
+}
+
+// Grows a row x column tile over the dimensions it leaves at t == 1, in A
+// order, each by its largest divisor that keeps the volume within vmax (the
+// column group's partial dimension, if any, must stay its outermost).
+static void rc_grow(const Geometry& g, int* tile, long long vmax) {
+  long long V = 1;
+  for (int k = 0; k < g.rank; ++k) V *= tile[k];
+  int na, nb, ga[kMaxDim], gb[kMaxDim];
+  if (!rc_groups(g, tile, na, ga, nb, gb)) return;
+  if (g.size[gb[nb - 1]] % tile[gb[nb - 1]] != 0) return;
+  for (int k = 0; k < g.rank; ++k) {
+    if (tile[k] != 1 || g.size[k] <= 1) continue;
+    const long long room = vmax / V;
+    if (room < 2) break;
+    int best = 1;
+    for (long long t = std::min<long long>(room, g.size[k]); t >= 2; --t)
+      if (g.size[k] % t == 0) {
+        best = (int)t;
+        break;
+      }
+    tile[k] = best;
+    V *= best;
+  }
 }
 
 ttc_status_t build_rc_params(const Geometry& g, const int* tile, int ta, int tb, int va, int vb, int grid_order,
@@ -614,8 +644,15 @@ bool choose_rc_tile(const Geometry& g, int ta, int tb, int va, int vb, int run_a
     for (int rb = 1; rb <= vmax; rb *= 2) {
       if ((long long)ra * rb > 4LL * vmax) continue;
       tile_for_runs(g, va, vb, vmax, szA, szB, run_a > 0 ? run_a : ra, run_b > 0 ? run_b : rb, cand);
+      rc_grow(g, cand, vmax);
       if (build_rc_params(g, cand, ta, tb, va, vb, 0, 1, p, lds) != TTC_SUCCESS) continue;
-      const double sc = tile_score(g, cand, szA, szB);
+      // Every tile slot is issued whether or not it holds an element: a
+      // mostly empty tile costs nearly a full one (measured: tiles filling
+      // 1% of the slots ran ~50x slower per byte, half-full ones ~15%).
+      long long V = 1;
+      for (int k = 0; k < g.rank; ++k) V *= cand[k];
+      const double u = (double)V / (double)vmax;
+      const double sc = tile_score(g, cand, szA, szB) * (u / (0.3 + 0.7 * u));
       if (sc > best * (1.0 + 1e-9)) {
         best = sc;
         for (int k = 0; k < kMaxDim; ++k) tile[k] = cand[k];
@@ -671,11 +708,19 @@ static ttc_status_t plan_fill(ttc_plan_s* pl, int va, int vb, const int* tile_in
     const bool rc_ok = !fp8 && fam != kFamilyBox &&
                        choose_rc_tile(g, pl->type_a, pl->type_b, va, vb, run_a, run_b, rc_tile, &rc_score);
     if (fam == kFamilyRC && !rc_ok) return TTC_ERROR_UNSUPPORTED;
-    if (fam == 0) fam = rc_ok ? kFamilyRC : kFamilyBox;
+    if (fam != kFamilyRC) choose_tile(g, pl->type_a, pl->type_b, va, vb, run_a, run_b, tile);
+    if (fam == 0) {
+      // The row x column kernel issues all of a tile's slots: it wins when
+      // its tile fills at least half of them or is no smaller than the box.
+      long long vr = 1, vbox = 1;
+      for (int k = 0; k < g.rank; ++k) {
+        vr *= rc_ok ? rc_tile[k] : 1;
+        vbox *= tile[k];
+      }
+      fam = rc_ok && (2 * vr >= max_rc_volume(pl->type_a, pl->type_b) || vr >= vbox) ? kFamilyRC : kFamilyBox;
+    }
     if (fam == kFamilyRC)
       for (int k = 0; k < kMaxDim; ++k) tile[k] = rc_tile[k];
-    else
-      choose_tile(g, pl->type_a, pl->type_b, va, vb, run_a, run_b, tile);
   }
   if (fam == kFamilyRC && fp8) return TTC_ERROR_UNSUPPORTED;
   pl->va = va;
@@ -792,7 +837,7 @@ extern "C" ttc_status_t ttc_plan_get_info(ttc_plan_t pl, ttc_plan_info_t* info) 
   }
   info->vec_a = pl->va;
   info->vec_b = pl->vb;
-  info->threads = kThreads;
+  info->threads = pl->family == kFamilyRC ? kRcThreads : kThreads;
   info->slots_a = (pl->p.V / pl->va + kThreads - 1) / kThreads;
   info->slots_b = (pl->p.V / pl->vb + kThreads - 1) / kThreads;
   info->tiles = pl->p.ntiles;

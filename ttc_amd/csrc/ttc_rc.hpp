@@ -38,8 +38,8 @@ struct RCWorker {
   using L = typename OP::L;
   static constexpr int NIA = rc_elems((int)sizeof(TA)) / VA;  // read vectors per thread per tile
   static constexpr int NIB = rc_elems((int)sizeof(TB)) / VB;  // write vectors per thread per tile
-  static constexpr uint32_t SA = kThreads * VA;               // read-walk step between a thread's slots
-  static constexpr uint32_t SB = kThreads * VB;               // write-walk step
+  static constexpr uint32_t SA = kRcThreads * VA;               // read-walk step between a thread's slots
+  static constexpr uint32_t SB = kRcThreads * VB;               // write-walk step
 
   L* data;              // LDS tile
   const int32_t* rowA;  // LDS tables, byte offsets
@@ -71,7 +71,7 @@ struct RCWorker {
     int32_t* rB = tab + p.R;
     int32_t* cA = tab + 2 * p.R;
     int32_t* cB = tab + 2 * p.R + p.C;
-    for (int i = (int)tid; i < p.R; i += kThreads) {
+    for (int i = (int)tid; i < p.R; i += kRcThreads) {
       uint32_t l = (uint32_t)i;
       int32_t oa = 0, ob = 0;
       for (int m = 0; m < p.na; ++m) {
@@ -84,7 +84,7 @@ struct RCWorker {
       rA[i] = oa;
       rB[i] = ob;
     }
-    for (int j = (int)tid; j < p.C; j += kThreads) {
+    for (int j = (int)tid; j < p.C; j += kRcThreads) {
       uint32_t l = (uint32_t)j;
       int32_t oa = 0, ob = 0;
       for (int m = 0; m < p.nb; ++m) {
@@ -135,6 +135,46 @@ struct RCWorker {
     c.cols = lastB ? (uint32_t)p.Cl : (uint32_t)p.C;
     const FastDiv dr = lastA ? p.dRl : p.dR;
     const FastDiv dc = lastB ? p.dCl : p.dC;
+    finish(c, dr, dc, empty);
+    return c;
+  }
+
+  // Offset tables copied from host-computed ones (grouped launches:
+  // rowA[R], rowB[R], colA[C], colB[C] in `tab`).  The caller has passed a
+  // barrier since the previous tile's last LDS read.
+  __device__ __forceinline__ void setup_tables(const RCParams& p, const int32_t* __restrict__ tab,
+                                               unsigned char* lds_raw, uint32_t tid_) {
+    tid = tid_;
+    data = reinterpret_cast<L*>(lds_raw);
+    int32_t* t = reinterpret_cast<int32_t*>(lds_raw + (size_t)p.lds_data_elems * sizeof(L));
+    const int n = 2 * p.R + 2 * p.C;
+    for (int i = (int)tid; i < n; i += kRcThreads) t[i] = tab[i];
+    rowA = t;
+    rowB = t + p.R;
+    colA = t + 2 * p.R;
+    colB = t + 2 * p.R + p.C;
+    __syncthreads();
+  }
+
+  // Tile from a host-computed record (grouped launches): origin offsets and
+  // whether it is the last, partial tile along the row / column group.
+  template <class Rec>
+  __device__ __forceinline__ Tile tile_rec(const RCParams& p, Rec& rec, bool empty) const {
+    Tile c;
+    c.bA = uniform64(rec.a);
+    c.bB = uniform64(rec.b);
+    const uint32_t f = uniform(rec.flags);
+    const bool lastA = (f & 1u) != 0, lastB = (f & 2u) != 0;
+    c.rows = lastA ? (uint32_t)p.Rl : (uint32_t)p.R;
+    c.cols = lastB ? (uint32_t)p.Cl : (uint32_t)p.C;
+    const FastDiv dr = lastA ? p.dRl : p.dR;
+    const FastDiv dc = lastB ? p.dCl : p.dC;
+    finish(c, dr, dc, empty);
+    return c;
+  }
+
+  // Per-lane walk starts and the uniform steps of a tile whose shape is set.
+  __device__ __forceinline__ void finish(Tile& c, const FastDiv& dr, const FastDiv& dc, bool empty) const {
     const int32_t n = empty ? 0 : (int32_t)(c.rows * c.cols);
     c.limA = n - (int32_t)(tid * VA);
     c.limB = n - (int32_t)(tid * VB);
@@ -146,7 +186,6 @@ struct RCWorker {
     c.bj = tid * VB - c.bi * c.cols;
     c.bdi = fdiv(SB, dc);
     c.bdj = SB - c.bdi * c.cols;
-    return c;
   }
 
   template <bool DENSE>
@@ -273,10 +312,39 @@ struct RCWorker {
   }
 };
 
+// Host-computed tile records read through the constant address space
+// (scalar loads; the table does not change during a launch).
+using KTile = const __attribute__((address_space(4))) RCTile;
+
+// run() over tiles [lo, hi) of a host-computed record table.
+template <class OP, int VA, int VB, bool BETA, bool NT>
+__device__ __forceinline__ void rc_run_table(RCWorker<OP, VA, VB, BETA, NT>& w, const RCParams& p,
+                                             KTile* recs, uint32_t lo, uint32_t hi,
+                                             const typename OP::TA* __restrict__ A, typename OP::TB* __restrict__ B,
+                                             typename OP::AL alpha, typename OP::BE beta) {
+  using W = RCWorker<OP, VA, VB, BETA, NT>;
+  if (lo >= hi) return;
+  typename OP::TA va[W::NIA][VA];
+  typename OP::TB ob[W::NIB][VB];
+  typename W::Tile c = w.tile_rec(p, recs[lo], false);
+  w.load(p, c, A, va);
+  for (uint32_t t = lo; t < hi; ++t) {
+    w.stage(p, c, va, alpha);
+    __syncthreads();
+    if constexpr (BETA) w.load_b(p, c, B, ob);
+    const uint32_t tn = t + 1 < hi ? t + 1 : t;
+    const typename W::Tile cn = w.tile_rec(p, recs[tn], t + 1 >= hi);
+    w.load(p, cn, A, va);
+    w.store(p, c, B, beta, ob);
+    __syncthreads();
+    c = cn;
+  }
+}
+
 // The parameters are the kernel's first argument, so they sit at the start
 // of the kernel-argument segment.
 template <class OP, int VA, int VB, bool BETA, bool NT>
-__global__ void __launch_bounds__(kThreads) transpose_rc_kernel(const RCParams p, const typename OP::TA* __restrict__ A,
+__global__ void __launch_bounds__(kRcThreads) transpose_rc_kernel(const RCParams p, const typename OP::TA* __restrict__ A,
                                                                 typename OP::TB* __restrict__ B, double alpha,
                                                                 double beta) {
   extern __shared__ __align__(16) unsigned char lds_raw[];

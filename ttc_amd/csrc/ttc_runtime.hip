@@ -52,14 +52,16 @@ int device_cus(int dev) {
   return cus;
 }
 
-int occupancy(const void* func, int lds_bytes) {
+int occupancy(const void* func, int lds_bytes, int threads) {
   static std::map<std::pair<const void*, int>, int> cache;
   std::lock_guard<std::mutex> lk(g_mu);
   auto key = std::make_pair(func, lds_bytes);
   auto it = cache.find(key);
   if (it != cache.end()) return it->second;
+  // Work-groups above 64 KiB of LDS must say so once per kernel (gfx950 has 160 KiB per CU).
+  if (lds_bytes > 65536) (void)hipFuncSetAttribute(func, hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes);
   int n = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, func, kThreads, (size_t)lds_bytes) != hipSuccess || n <= 0)
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, func, threads, (size_t)lds_bytes) != hipSuccess || n <= 0)
     n = 1;
   cache[key] = n;
   return n;
@@ -70,12 +72,13 @@ int occupancy(const void* func, int lds_bytes) {
 // occupancy limit (5-9), which only adds concurrent tile streams.
 static constexpr int kDefaultWgPerCu = 4;
 
-unsigned grid_size(uint32_t ntiles, int wg_per_cu, const void* func, int lds_bytes) {
+unsigned grid_size(uint32_t ntiles, int wg_per_cu, const void* func, int lds_bytes, int threads) {
   int dev = 0;
   (void)hipGetDevice(&dev);
   const int cus = device_cus(dev);
-  const int occ = occupancy(func, lds_bytes);
-  int wg = wg_per_cu > 0 ? wg_per_cu : (occ < kDefaultWgPerCu ? occ : kDefaultWgPerCu);
+  const int occ = occupancy(func, lds_bytes, threads);
+  const int def = kDefaultWgPerCu * kThreads / threads;
+  int wg = wg_per_cu > 0 ? wg_per_cu : (occ < def ? occ : def);
   if (wg < 1) wg = 1;
   const uint64_t g = (uint64_t)cus * (uint64_t)wg;
   return (unsigned)(ntiles < g ? ntiles : g);
@@ -101,7 +104,8 @@ ttc_status_t launch_plan(const ttc_plan_s* pl, const void* A, void* B, double al
   const int mode = pl->beta_is_zero ? (pl->nt_store ? kModeBzNt : kModeBz) : kModeBeta;
   const KernelEntry* e = find_kernel(pl->type_a, pl->type_b, use->va, use->vb, mode, use->family);
   if (!e) return TTC_ERROR_UNSUPPORTED;
-  const unsigned grid = grid_size(use->p.ntiles, pl->wg_per_cu, e->func, use->lds_bytes);
+  const unsigned grid = grid_size(use->p.ntiles, pl->wg_per_cu, e->func, use->lds_bytes,
+                                  use->family == kFamRC ? kRcThreads : kThreads);
   const hipError_t err = use->family == kFamRC
                              ? e->launch_rc(use->rc, A, B, alpha, beta, grid, (size_t)use->lds_bytes, s)
                              : e->launch(use->p, A, B, alpha, beta, grid, (size_t)use->lds_bytes, s);
@@ -198,7 +202,7 @@ extern "C" ttc_status_t ttc_execute_fp8(ttc_plan_t pl, const void* A, void* B, d
     amax_out = nullptr;
     d_scale_in = nullptr;
   }
-  const unsigned grid = grid_size(use->p.ntiles, pl->wg_per_cu, e->func, use->lds_bytes);
+  const unsigned grid = grid_size(use->p.ntiles, pl->wg_per_cu, e->func, use->lds_bytes, kThreads);
   return e->launch_fp8(use->p, A, B, al, d_scale_in, amax_in, fp8_max, scale, amax_out, d_scale_inv_out, grid,
                        (size_t)use->lds_bytes, s) == hipSuccess
              ? TTC_SUCCESS
