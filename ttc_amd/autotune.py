@@ -3,10 +3,10 @@
 TTC's method — enumerate blockings x loop orders, prune to about
 sqrt(maxImplementations) of each by cost models, compile and time every
 survivor, keep the fastest (ttc/GPUtransposeGenerator.py:176-230, 328-349;
-ttc/ttc.py:871-1022) — is kept, with the search space of this kernel: tile
-shape (through run-length targets on A's and B's side), vector width per
-side, persistent work-groups per CU, tile-walk order and non-temporal
-stores.  Nothing is compiled per candidate: every candidate is a launch of a
+ttc/ttc.py:871-1022) — is kept, with the search space of these kernels:
+kernel family (row x column or box tiles), tile shape (through run-length
+targets on A's and B's side), vector width per side, tile numbering,
+tile-walk order and non-temporal stores.  Nothing is compiled per candidate: every candidate is a launch of a
 prebuilt kernel with different arguments, timed with HIP events.
 """
 import math
@@ -15,20 +15,27 @@ from . import runtime as rt
 
 
 def tile_score(info, size_a, size_b):
-    """Same model as the planner (csrc/ttc_plan.cpp tile_score): fraction of
-    useful tile slots x b/(b+64) per side for contiguous runs of b bytes x
-    V/(V+256)."""
+    """Same model as the planner (csrc/ttc_plan.cpp tile_score and
+    choose_rc_tile): fraction of useful tile slots x b/(b+64) per side for
+    contiguous runs of b bytes x V/(V+256), times u/(0.3+0.7u) for the
+    row x column kernel, whose tiles fill a fraction u of its slots."""
     n = info["elements"]
     if n == 0:
         return 0.0
     eff = n / float(info["tiles"] * info["tile_volume"])
     ba, bb = info["run_a"] * size_a, info["run_b"] * size_b
     v = info["tile_volume"]
-    return eff * (ba / (ba + 64.0)) * (bb / (bb + 64.0)) * (v / (v + 256.0))
+    sc = eff * (ba / (ba + 64.0)) * (bb / (bb + 64.0)) * (v / (v + 256.0))
+    if info["family"] == 2:
+        cap = info["threads"] * min(32, 128 // max(size_a, size_b))
+        u = v / float(cap)
+        sc *= u / (0.3 + 0.7 * u)
+    return sc
 
 
 def candidates(type_a, type_b, perm, size, lda=None, ldb=None, beta_is_zero=False, max_impl=200):
-    """List of config dicts, best-first by the model, at most max_impl."""
+    """List of config dicts, best-first by the model, at most max_impl.  The
+    planner's own choice ({}) always comes first."""
     base = rt.Plan(type_a, type_b, perm, size, lda, ldb, beta_is_zero).info()
     sa, sb = rt.DTYPE_BYTES[type_a], rt.DTYPE_BYTES[type_b]
     vecs = []
@@ -38,35 +45,41 @@ def candidates(type_a, type_b, perm, size, lda=None, ldb=None, beta_is_zero=Fals
                 vecs.append((va, vb))
     tiles = {}
     runs = [1 << k for k in range(0, 14)]
-    for va, vb in vecs:
-        for ra in runs:
-            for rb in runs:
-                if ra * rb > 4 * 16384:
-                    continue
-                cfg = {"vec_a": va, "vec_b": vb, "run_a": ra, "run_b": rb}
-                try:
-                    info = rt.Plan(type_a, type_b, perm, size, lda, ldb, beta_is_zero, cfg).info()
-                except rt.TTCError:
-                    continue
-                key = (tuple(info["tile"]), va, vb)
-                if key not in tiles:
-                    tiles[key] = (tile_score(info, sa, sb), {"tile": info["tile"], "vec_a": va, "vec_b": vb})
+    for fam in (2, 1):
+        for va, vb in vecs:
+            for ra in runs:
+                for rb in runs:
+                    if ra * rb > 4 * 16384:
+                        continue
+                    cfg = {"vec_a": va, "vec_b": vb, "run_a": ra, "run_b": rb, "family": fam}
+                    try:
+                        info = rt.Plan(type_a, type_b, perm, size, lda, ldb, beta_is_zero, cfg).info()
+                    except rt.TTCError:
+                        continue
+                    key = (fam, tuple(info["tile"]), va, vb)
+                    if key not in tiles:
+                        tiles[key] = (tile_score(info, sa, sb),
+                                      {"tile": info["tile"], "vec_a": va, "vec_b": vb, "family": fam})
     ranked = [c for _, c in sorted(tiles.values(), key=lambda x: -x[0])]
+    # launch variants: tile numbering (row x column tiles), tile walk, store mode
     launch = []
-    for wg in (0, 2, 4):
-        for order in (1, 2):
-            for nt in ((1, 2) if beta_is_zero else (0,)):
-                launch.append({"wg_per_cu": wg, "grid_order": order, "nt_store": nt})
+    for order in (3, 1):
+        for tord in (2, 1):
+            for nt in ((2, 1) if beta_is_zero else (0,)):
+                launch.append({"grid_order": order, "tile_order": tord, "nt_store": nt})
     # sqrt pruning of both axes, as the reference combines its best blockings
     # with its best loop orders
-    k = max(1, int(math.ceil(math.sqrt(max_impl))))
-    while len(ranked) * len(launch) > max_impl and len(ranked) > k:
+    budget = max(1, max_impl - 1)
+    k = max(1, int(math.ceil(math.sqrt(budget))))
+    while len(ranked) * len(launch) > budget and len(ranked) > k:
         ranked.pop()
-    while len(ranked) * len(launch) > max_impl and len(launch) > 1:
+    while len(ranked) * len(launch) > budget and len(launch) > 1:
         launch.pop()
-    out = []
+    out = [{}]
     for t in ranked:
         for l in launch:
+            if t["family"] == 1 and l["tile_order"] != 2:
+                continue  # box tiles have one numbering
             c = dict(t)
             c.update(l)
             out.append(c)

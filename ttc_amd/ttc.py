@@ -345,7 +345,8 @@ def generateTransposition(ttcArgs):
                 results[0]["GiBs"], results[0]["GBs"]))
     if best_cfg is None:  # --generateOnly without a stored solution: the planner's choice
         info = base_plan.info()
-        best_cfg = {"tile": info["tile"], "vec_a": info["vec_a"], "vec_b": info["vec_b"]}
+        best_cfg = {"tile": info["tile"], "vec_a": info["vec_a"], "vec_b": info["vec_b"], "family": info["family"],
+                    "tile_order": info["tile_order"]}
 
     # ----------------------------------------------------- emit artefact
     directory = os.path.join(os.getcwd(), "ttc_transpositions")
