@@ -61,6 +61,13 @@ def candidates(type_a, type_b, perm, size, lda=None, ldb=None, beta_is_zero=Fals
                         tiles[key] = (tile_score(info, sa, sb),
                                       {"tile": info["tile"], "vec_a": va, "vec_b": vb, "family": fam})
     ranked = [c for _, c in sorted(tiles.values(), key=lambda x: -x[0])]
+    # the best tile of each family first, so that pruning never drops a family
+    heads = []
+    for fam in (2, 1):
+        best = next((c for c in ranked if c["family"] == fam), None)
+        if best is not None:
+            heads.append(best)
+    ranked = heads + [c for c in ranked if all(c is not h for h in heads)]
     # launch variants: tile numbering (row x column tiles), tile walk, store mode
     launch = []
     for order in (3, 1):
