@@ -8,6 +8,11 @@ B[pi(i)] = alpha*A[i] + beta*B[pi(i)], inputs resident in HBM.  All beta = 0
 ops run first, then all beta = 1 ops, so consecutive ops never touch the same
 buffers and no op finds its data in the 256 MB last-level cache.
 
+Plans: by default each op is autotuned first (at most 12 candidates timed
+on its own buffers, outside the timed region), as TTC generates and measures
+variants before use (ttc/ttc.py:871-1022); the planner's untuned choice is
+timed in the same run and reported beside it.
+
 Metric: GB/s (10^9 B/s) of algorithmic traffic, N*sA + N*sB (+ N*sB when
 beta != 0) (ttc/transposeGenerator.py:665-668), over the whole timed region,
 plus a roofline line for the dominant kernel and the oracle's CPU rate on a
@@ -50,7 +55,10 @@ def kernel_name(info):
                                             "true" if info["nt_store"] else "false")
 
 
-def build_c2(torch, rt, wl, dev):
+def build_c2(torch, rt, wl, dev, tune=0):
+    """The 64 ops of one step.  tune > 0: each op's configuration comes from
+    the autotuner (at most `tune` candidates timed on its own buffers before
+    the timed region, TTC's generate-then-run flow, ttc/ttc.py:871-1022)."""
     ops = []
     gen = torch.Generator(device=dev)
     gen.manual_seed(0)
@@ -75,9 +83,15 @@ def build_c2(torch, rt, wl, dev):
             B = torch.rand(nb, generator=gen, device=dev, dtype=tdt) * 2 - 1
             bufs[(case["k"], es)] = (A, B)
             for bz in (True, False):
-                plan = rt.Plan(t, t, case["perm"], size, lda, ldb, beta_is_zero=bz)
+                cfg = None
+                if tune > 0:
+                    from ttc_amd import autotune
+                    res = autotune.tune(t, t, case["perm"], size, lda, ldb, 1.0, 0.0 if bz else 1.0, tune, reps=5)
+                    cfg = res[0]["config"] if res else None
+                plan = rt.Plan(t, t, case["perm"], size, lda, ldb, beta_is_zero=bz, config=cfg)
+                plan0 = rt.Plan(t, t, case["perm"], size, lda, ldb, beta_is_zero=bz)
                 info = plan.info()
-                ops.append({"k": case["k"], "es": es, "bz": bz, "plan": plan, "A": A, "B": B,
+                ops.append({"k": case["k"], "es": es, "bz": bz, "plan": plan, "plan0": plan0, "A": A, "B": B,
                             "alpha": 1.0, "beta": 0.0 if bz else 1.0, "perm": case["perm"], "size": size,
                             "lda": lda, "ldb": ldb, "bytes": info["bytes_algorithmic"], "info": info,
                             "kernel": kernel_name(info)})
@@ -112,16 +126,16 @@ def verify_sample(torch, op, nsample=4096):
     return bool(torch.equal(a, b))
 
 
-def run_c2(torch, ops, stream, steps, warmup):
+def run_c2(torch, ops, stream, steps, warmup, key="plan"):
     # sample check of every beta = 0 op before the beta = 1 ops reuse B
     for o in ops:
         if o["bz"]:
-            o["plan"].execute(o["A"], o["B"], o["alpha"], o["beta"], stream=stream)
+            o[key].execute(o["A"], o["B"], o["alpha"], o["beta"], stream=stream)
     torch.cuda.synchronize()
     ok = all(verify_sample(torch, o) for o in ops if o["bz"])
     for _ in range(warmup):
         for o in ops:
-            o["plan"].execute(o["A"], o["B"], o["alpha"], o["beta"], stream=stream)
+            o[key].execute(o["A"], o["B"], o["alpha"], o["beta"], stream=stream)
     torch.cuda.synchronize()
     ev = [[(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in ops]
           for _ in range(steps)]
@@ -133,7 +147,7 @@ def run_c2(torch, ops, stream, steps, warmup):
     for s in range(steps):
         for i, o in enumerate(ops):
             ev[s][i][0].record(stream)
-            o["plan"].execute(o["A"], o["B"], o["alpha"], o["beta"], stream=stream)
+            o[key].execute(o["A"], o["B"], o["alpha"], o["beta"], stream=stream)
             ev[s][i][1].record(stream)
     t1.record(stream)
     torch.cuda.synchronize()
@@ -271,6 +285,9 @@ def main():
     ap.add_argument("--no-batch", dest="batch", action="store_false")
     ap.add_argument("--batch-replays", type=int, default=200)
     ap.add_argument("--calibrate-copy", action="store_true", help="run only the plain copy used to scale counters")
+    ap.add_argument("--tune", type=int, default=12,
+                    help="autotune each op over at most N candidates before timing, as TTC generates the fastest "
+                         "variant before use (0: the planner's choice only)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only; --gpus %d is not supported (a transposition shards as "
@@ -292,9 +309,13 @@ def main():
     rt.lib()
     stream = torch.cuda.Stream(device=dev)
     with torch.cuda.stream(stream):
-        ops, bufs = build_c2(torch, rt, wl, dev)
+        ops, bufs = build_c2(torch, rt, wl, dev, args.tune)
     torch.cuda.synchronize()
     log("c2: %d ops per step" % len(ops))
+    value_planner = None
+    if args.tune > 0:  # the planner's own choice, for comparison
+        t0_ms, _, _, ok0 = run_c2(torch, ops, stream, args.steps, args.warmup, key="plan0")
+        value_planner = sum(o["bytes"] for o in ops) * args.steps / (t0_ms * 1e-3) / 1e9
     total_ms, wall, per_op, ok = run_c2(torch, ops, stream, args.steps, args.warmup)
     step_bytes = sum(o["bytes"] for o in ops)
     value = step_bytes * args.steps / (total_ms * 1e-3) / 1e9
@@ -325,6 +346,8 @@ def main():
         "vs_baseline": None, "dtype": "f32,f64", "data": "synthetic",
         "config": {"workload": "16 seeded random perms rank 2-6, 200 MiB A each, fp32+fp64, beta 0 and 1 "
                                "(64 transposes per step)", "bytes_per_step": step_bytes,
+                   "plans": ("autotuned, <= %d candidates per op" % args.tune) if args.tune else "planner heuristic",
+                   "value_planner_heuristic": round(value_planner, 2) if value_planner else None,
                    "pct_of_hbm_peak": round(100.0 * value / HBM_PEAK_GBS, 2), "verified_sample": ok},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom),

@@ -33,9 +33,12 @@ def tile_score(info, size_a, size_b):
     return sc
 
 
-def candidates(type_a, type_b, perm, size, lda=None, ldb=None, beta_is_zero=False, max_impl=200):
+def candidates(type_a, type_b, perm, size, lda=None, ldb=None, beta_is_zero=False, max_impl=200,
+               nt_stores=False):
     """List of config dicts, best-first by the model, at most max_impl.  The
-    planner's own choice ({}) always comes first."""
+    planner's own choice ({}) always comes first.  Non-temporal stores are
+    tried only with nt_stores: they measure well in a loop over a few
+    buffers and lose in longer streams of different transpositions."""
     base = rt.Plan(type_a, type_b, perm, size, lda, ldb, beta_is_zero).info()
     sa, sb = rt.DTYPE_BYTES[type_a], rt.DTYPE_BYTES[type_b]
     vecs = []
@@ -72,7 +75,7 @@ def candidates(type_a, type_b, perm, size, lda=None, ldb=None, beta_is_zero=Fals
     launch = []
     for order in (3, 1):
         for tord in (2, 1):
-            for nt in ((2, 1) if beta_is_zero else (0,)):
+            for nt in ((2, 1) if beta_is_zero and nt_stores else (0,)):
                 launch.append({"grid_order": order, "tile_order": tord, "nt_store": nt})
     # sqrt pruning of both axes, as the reference combines its best blockings
     # with its best loop orders
@@ -103,15 +106,15 @@ def measure(plan, pairs, alpha, beta, reps=3, stream=None):
     right before a single timed launch would charge that flush's write-back
     to the kernel.)"""
     import torch
-    s = stream or torch.cuda.current_stream()
+    s = stream or torch.cuda.current_stream()  # launches and events on the same stream
     for A, B in pairs:
-        plan.execute(A, B, alpha, beta, stream=stream)
+        plan.execute(A, B, alpha, beta, stream=s)
     rounds = []
     for _ in range(reps):
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record(s)
         for A, B in pairs:
-            plan.execute(A, B, alpha, beta, stream=stream)
+            plan.execute(A, B, alpha, beta, stream=s)
         e1.record(s)
         e1.synchronize()
         rounds.append(e0.elapsed_time(e1) / len(pairs))
@@ -131,7 +134,7 @@ def _device_inputs(torch, type_a, type_b, perm, size, lda, ldb, dev):
 
 
 def tune(type_a, type_b, perm, size, lda=None, ldb=None, alpha=1.0, beta=0.0, max_impl=200, reps=5,
-         progress=None):
+         progress=None, nt_stores=False):
     """Times every candidate on device buffers of the problem's shape.
     Returns results sorted fastest first: dicts with config, info, ms, GBs
     (10^9 B/s) and GiBs (2^30 B/s, the reference's unit)."""
@@ -140,12 +143,13 @@ def tune(type_a, type_b, perm, size, lda=None, ldb=None, alpha=1.0, beta=0.0, ma
     bz = beta == 0
     A, B = _device_inputs(torch, type_a, type_b, perm, size, lda, ldb, dev)
     pairs = [(A, B)]
-    # enough distinct buffers that one round does not fit the 256 MB cache
+    # enough distinct buffers that a round streams well past the 256 MB
+    # last-level cache, and at least four launches per round
     footprint = A.numel() + B.numel()
-    while footprint * len(pairs) < (768 << 20) and len(pairs) < 16:
+    while (footprint * len(pairs) < (1536 << 20) or len(pairs) < 4) and len(pairs) < 16:
         pairs.append(_device_inputs(torch, type_a, type_b, perm, size, lda, ldb, dev))
     results = []
-    cands = candidates(type_a, type_b, perm, size, lda, ldb, bz, max_impl)
+    cands = candidates(type_a, type_b, perm, size, lda, ldb, bz, max_impl, nt_stores)
     for i, cfg in enumerate(cands):
         try:
             plan = rt.Plan(type_a, type_b, perm, size, lda, ldb, bz, cfg)

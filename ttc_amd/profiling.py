@@ -3,6 +3,9 @@
   python -m ttc_amd.profiling stats DIR [-o OUT.md]
       per-kernel launches, total and average duration from a
       `rocprofv3 --kernel-trace --stats` output directory.
+  python -m ttc_amd.profiling counters DIR [-o OUT.md]
+      per-kernel average of every counter of a `rocprofv3 --pmc` run
+      (for example SQ_LDS_BANK_CONFLICT against SQ_ACTIVE_INST_LDS).
   python -m ttc_amd.profiling traffic --fetch DIR --write DIR
          --calib-fetch DIR --calib-write DIR --calib-bytes N [-o OUT]
       HBM bytes per launch of every kernel from two `rocprofv3 --pmc` runs
@@ -106,6 +109,10 @@ def main(argv=None):
     s.add_argument("dir")
     s.add_argument("-o", "--out")
     s.add_argument("--command", default="", help="the command that produced DIR (recorded in the summary)")
+    c = sub.add_parser("counters")
+    c.add_argument("dir")
+    c.add_argument("-o", "--out")
+    c.add_argument("--command", default="", help="the command that produced DIR (recorded in the summary)")
     t = sub.add_parser("traffic")
     for a in ("--fetch", "--write", "--calib-fetch", "--calib-write"):
         t.add_argument(a, required=True)
@@ -120,6 +127,29 @@ def main(argv=None):
                  "| kernel | launches | total ms | avg us | % |", "|---|---:|---:|---:|---:|"]
         for name, calls, tot, avg in rows:
             lines.append("| `%s` | %d | %.3f | %.2f | %.1f |" % (name, calls, tot / 1e6, avg / 1e3, 100 * tot / total))
+        text = "\n".join(lines) + "\n"
+        if args.out:
+            with open(args.out, "w") as f:
+                f.write(text)
+        sys.stdout.write(text)
+        return 0
+    if args.cmd == "counters":
+        per = {}
+        with open(_one(args.dir, "*counter_collection.csv")) as f:
+            for r in csv.DictReader(f):
+                k = short_name(r["Kernel_Name"])
+                if "ttc::" not in k:
+                    continue
+                per.setdefault(k, {}).setdefault(r["Counter_Name"], []).append(float(r["Counter_Value"]))
+        names = sorted({n for v in per.values() for n in v})
+        lines = ["# rocprofv3 counters per kernel (average per dispatch)", ""]
+        if args.command:
+            lines += ["Command: `%s`" % args.command, ""]
+        lines += ["| kernel | dispatches | " + " | ".join(names) + " |", "|---|---:|" + "---:|" * len(names)]
+        for k, v in sorted(per.items()):
+            n = len(next(iter(v.values())))
+            lines.append("| `%s` | %d | " % (k, n) + " | ".join(
+                "%.4g" % (sum(v[x]) / len(v[x])) if x in v else "" for x in names) + " |")
         text = "\n".join(lines) + "\n"
         if args.out:
             with open(args.out, "w") as f:
