@@ -127,6 +127,10 @@ def verify_sample(torch, op, nsample=4096):
 
 
 def run_c2(torch, ops, stream, steps, warmup, key="plan"):
+    """Times `steps` passes over the ops with one pair of events around the
+    whole region; nothing else is enqueued between the launches (an event
+    recorded between two kernels adds a marker packet and costs several
+    microseconds per op)."""
     # sample check of every beta = 0 op before the beta = 1 ops reuse B
     for o in ops:
         if o["bz"]:
@@ -136,25 +140,52 @@ def run_c2(torch, ops, stream, steps, warmup, key="plan"):
     for _ in range(warmup):
         for o in ops:
             o[key].execute(o["A"], o["B"], o["alpha"], o["beta"], stream=stream)
-    torch.cuda.synchronize()
-    ev = [[(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in ops]
-          for _ in range(steps)]
     t0 = torch.cuda.Event(enable_timing=True)
     t1 = torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     wall0 = time.perf_counter()
     t0.record(stream)
-    for s in range(steps):
-        for i, o in enumerate(ops):
-            ev[s][i][0].record(stream)
+    for _ in range(steps):
+        for o in ops:
             o[key].execute(o["A"], o["B"], o["alpha"], o["beta"], stream=stream)
-            ev[s][i][1].record(stream)
     t1.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - wall0
-    total_ms = t0.elapsed_time(t1)
-    per_op_ms = [[ev[s][i][0].elapsed_time(ev[s][i][1]) for i in range(len(ops))] for s in range(steps)]
-    return total_ms, wall, per_op_ms, ok
+    return t0.elapsed_time(t1), wall, ok
+
+
+def per_op_ms(torch, ops, stream, reps):
+    """[[ms of op i in rep r]]: each launch between its own pair of events
+    (diagnostic: the events add a few microseconds to each launch)."""
+    ev = [[(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in ops]
+          for _ in range(reps)]
+    torch.cuda.synchronize()
+    for r in range(reps):
+        for i, o in enumerate(ops):
+            ev[r][i][0].record(stream)
+            o["plan"].execute(o["A"], o["B"], o["alpha"], o["beta"], stream=stream)
+            ev[r][i][1].record(stream)
+    torch.cuda.synchronize()
+    return [[ev[r][i][0].elapsed_time(ev[r][i][1]) for i in range(len(ops))] for r in range(reps)]
+
+
+def kernel_avg_ms(torch, ops, stream, reps):
+    """Average launch duration of one kernel: its ops launched in turn, `reps`
+    rounds, with one pair of events around the whole region.  Consecutive
+    launches touch different tensors, so no launch finds its data in cache."""
+    for o in ops:
+        o["plan"].execute(o["A"], o["B"], o["alpha"], o["beta"], stream=stream)
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0.record(stream)
+    for _ in range(reps):
+        for o in ops:
+            o["plan"].execute(o["A"], o["B"], o["alpha"], o["beta"], stream=stream)
+    t1.record(stream)
+    torch.cuda.synchronize()
+    n = reps * len(ops)
+    return t0.elapsed_time(t1) / n, sum(o["bytes"] for o in ops) * reps // n
 
 
 def cpu_baseline(budget_s):
@@ -202,9 +233,11 @@ def cpu_baseline(budget_s):
                       "512 MiB cache-trash sweep" % (nthreads, model, ",".join(done))}
 
 
-def run_batch(torch, rt, wl, dev, stream, replays):
+def run_batch(torch, rt, wl, dev, stream, replays, tune=0):
     """512 mixed ops: grouped persistent launch (graph replay) vs the per-op
-    launch loop with the same kernels."""
+    launch loop with the same kernels.  tune > 0: every op of at least 1 MiB
+    takes the autotuner's configuration (at most `tune` candidates); the
+    planner's choices are timed beside it."""
     ops = wl.batch_c5()
     gen = torch.Generator(device=dev)
     gen.manual_seed(5)
@@ -236,18 +269,51 @@ def run_batch(torch, rt, wl, dev, stream, replays):
         return e0.elapsed_time(e1) / reps
 
     t_group = timed(lambda: batch.graph_launch(stream, per_op_loop=False), replays)
+    t_group_planner, tuned = None, 0
+    if tune > 0:
+        from ttc_amd import autotune
+        cfgs, tuned = autotune.tune_batch(desc, max_impl=tune)
+        tb = rt.Batch([dict(d, config=c) for d, c in zip(desc, cfgs)])
+        t_group_planner = t_group
+        if timed(lambda: tb.graph_launch(stream, per_op_loop=False), replays // 4 + 1) < t_group:
+            batch.close()
+            batch, info = tb, tb.info()  # the search's configurations, timed afresh below
+            t_group = timed(lambda: batch.graph_launch(stream, per_op_loop=False), replays)
+        else:
+            tb.close()
     t_group_eager = timed(lambda: batch.execute(stream), max(10, replays // 10))
     t_loop = timed(lambda: batch.execute_loop(stream), max(10, replays // 10))
     t_loop_graph = timed(lambda: batch.graph_launch(stream, per_op_loop=True), max(10, replays // 10))
+
+    # the reference wrapper's semantics: one synchronous call per op on the
+    # null stream (ttc/CUDAtranspose.py:395-429), here without its per-call
+    # metadata upload (plans are cached by shape)
+    def legacy_loop():
+        for d in desc:
+            rt.legacy_transpose(d["type_a"], d["type_b"], d["A"], d["B"], d["alpha"], d["beta"], d["perm"],
+                                d["size"], beta_is_zero=d["beta"] == 0)
+    legacy_loop()
+    torch.cuda.synchronize()
+    reps = 3
+    w0 = time.perf_counter()
+    for _ in range(reps):
+        legacy_loop()
+    t_sync = (time.perf_counter() - w0) * 1e3 / reps
     n = len(ops)
     return {"ops": n, "replays": replays, "tiles": info["tiles"], "workgroups": info["workgroups"],
             "bytes_per_launch": info["bytes"],
-            "grouped_graph_ms": round(t_group, 4), "grouped_eager_ms": round(t_group_eager, 4),
+            "configs": ("autotuned (<= %d candidates) for the %d distinct problems of >= 1 MiB, planner for the "
+                        "rest" % (tune, tuned)) if tune else "planner",
+            "grouped_graph_ms": round(t_group, 4),
+            "grouped_graph_ms_planner": round(t_group_planner, 4) if t_group_planner else None,
+            "grouped_eager_ms": round(t_group_eager, 4),
             "loop_async_ms": round(t_loop, 4), "loop_graph_ms": round(t_loop_graph, 4),
+            "loop_sync_legacy_ms": round(t_sync, 3),
             "transposes_per_s_grouped": round(n / (t_group * 1e-3), 1),
             "transposes_per_s_loop": round(n / (t_loop * 1e-3), 1),
             "speedup_vs_async_loop": round(t_loop / t_group, 3),
             "speedup_vs_graph_loop": round(t_loop_graph / t_group, 3),
+            "speedup_vs_sync_legacy_loop": round(t_sync / t_group, 3),
             "grouped_GBps": round(info["bytes"] / (t_group * 1e-3) / 1e9, 1)}
 
 
@@ -283,7 +349,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--batch", dest="batch", action="store_true", default=True)
     ap.add_argument("--no-batch", dest="batch", action="store_false")
-    ap.add_argument("--batch-replays", type=int, default=200)
+    ap.add_argument("--batch-replays", type=int, default=1000)
     ap.add_argument("--calibrate-copy", action="store_true", help="run only the plain copy used to scale counters")
     ap.add_argument("--tune", type=int, default=12,
                     help="autotune each op over at most N candidates before timing, as TTC generates the fastest "
@@ -314,31 +380,37 @@ def main():
     log("c2: %d ops per step" % len(ops))
     value_planner = None
     if args.tune > 0:  # the planner's own choice, for comparison
-        t0_ms, _, _, ok0 = run_c2(torch, ops, stream, args.steps, args.warmup, key="plan0")
+        t0_ms, _, _ = run_c2(torch, ops, stream, args.steps, args.warmup, key="plan0")
         value_planner = sum(o["bytes"] for o in ops) * args.steps / (t0_ms * 1e-3) / 1e9
-    total_ms, wall, per_op, ok = run_c2(torch, ops, stream, args.steps, args.warmup)
+    total_ms, wall, ok = run_c2(torch, ops, stream, args.steps, args.warmup)
     step_bytes = sum(o["bytes"] for o in ops)
     value = step_bytes * args.steps / (total_ms * 1e-3) / 1e9
-    # per-kernel aggregation (the roofline line is for the kernel with the most time)
+    # per-op and per-kernel times (diagnostic pass, each launch between events)
+    per_op = per_op_ms(torch, ops, stream, 3)
     groups = {}
-    for s in range(args.steps):
+    for r in per_op:
         for i, o in enumerate(ops):
-            g = groups.setdefault(o["kernel"], {"ms": 0.0, "bytes": 0, "launches": 0})
-            g["ms"] += per_op[s][i]
+            g = groups.setdefault(o["kernel"], {"ms": 0.0, "bytes": 0, "launches": 0, "ops": []})
+            g["ms"] += r[i]
             g["bytes"] += o["bytes"]
             g["launches"] += 1
-    dom = max(groups, key=lambda k: groups[k]["ms"])
-    gd = groups[dom]
-    achieved = gd["bytes"] / (gd["ms"] * 1e-3) / 1e9
+            if r is per_op[0]:
+                g["ops"].append(o)
     for i, o in enumerate(ops):
-        ms = min(per_op[s][i] for s in range(args.steps))
-        log("  k=%2d %s %s rank=%d tile=%s vec=%d/%d %8.1f GB/s  %.3f ms  %s" % (
+        t = sorted(r[i] for r in per_op)
+        log("  k=%2d %s %s rank=%d tile=%s vec=%d/%d %8.1f GB/s  min %.3f ms  median %.3f ms  %s" % (
             o["k"], "f32" if o["es"] == 4 else "f64", "b0" if o["bz"] else "b1", o["info"]["rank"],
-            o["info"]["tile"], o["info"]["vec_a"], o["info"]["vec_b"], o["bytes"] / (ms * 1e-3) / 1e9, ms,
-            o["kernel"]))
+            o["info"]["tile"], o["info"]["vec_a"], o["info"]["vec_b"], o["bytes"] / (t[0] * 1e-3) / 1e9, t[0],
+            t[len(t) // 2], o["kernel"]))
     for k, g in sorted(groups.items(), key=lambda kv: -kv[1]["ms"]):
         log("  kernel %-48s launches=%4d avg=%.4f ms  %.1f GB/s" % (
             k, g["launches"], g["ms"] / g["launches"], g["bytes"] / (g["ms"] * 1e-3) / 1e9))
+    # the roofline line is for the kernel with the most time, timed in a region of its own
+    dom = max(groups, key=lambda k: groups[k]["ms"])
+    avg_ms, bytes_per_launch = kernel_avg_ms(torch, groups[dom]["ops"], stream, max(2, args.steps))
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    log("  roofline kernel %s: %d ops x %d rounds, avg %.4f ms, %.1f GB/s" % (
+        dom, len(groups[dom]["ops"]), max(2, args.steps), avg_ms, achieved))
     out = {
         "metric": "GB/s (r+w) & % of HBM peak per transpose; batched transposes/sec vs plain",
         "value": round(value, 2), "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -348,17 +420,19 @@ def main():
                                "(64 transposes per step)", "bytes_per_step": step_bytes,
                    "plans": ("autotuned, <= %d candidates per op" % args.tune) if args.tune else "planner heuristic",
                    "value_planner_heuristic": round(value_planner, 2) if value_planner else None,
-                   "pct_of_hbm_peak": round(100.0 * value / HBM_PEAK_GBS, 2), "verified_sample": ok},
+                   "value_GiBps": round(value * 1e9 / 2 ** 30, 2),
+                   "pct_of_hbm_peak": round(100.0 * value / HBM_PEAK_GBS, 2), "verified_sample": ok,
+                   "timed_wall_s": round(wall, 3)},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom),
-                     "avg_launch_ms": round(gd["ms"] / gd["launches"], 5),
-                     "bytes_per_launch": gd["bytes"] // gd["launches"]},
+                     "avg_launch_ms": round(avg_ms, 5), "bytes_per_launch": bytes_per_launch,
+                     "launches": len(groups[dom]["ops"]) * max(2, args.steps)},
     }
     if args.batch:
         del bufs, ops
         torch.cuda.empty_cache()
         try:
-            out["batched"] = run_batch(torch, rt, wl, dev, stream, args.batch_replays)
+            out["batched"] = run_batch(torch, rt, wl, dev, stream, args.batch_replays, min(args.tune, 4))
         except Exception as e:  # reported, not hidden
             out["batched"] = {"error": str(e)}
     out["cpu_baseline"] = None if args.no_cpu else cpu_baseline(args.cpu_seconds)

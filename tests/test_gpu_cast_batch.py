@@ -177,6 +177,38 @@ def test_batch_graph_replays(gpu):
             junk.mul_(2.0).sum().item()
 
 
+def test_batch_with_tuned_configs_matches_oracle(gpu):
+    """Per-op configurations from the batch tuner (every op tuned here) give
+    the same bits, grouped and through a graph replay."""
+    import torch
+    from ttc_amd import autotune
+    from ttc_amd import runtime as rt
+    rng = np.random.default_rng(79)
+    ops = _batch_ops(rng, 24, [S, D, BF16])
+    for o in ops:
+        o["A"], o["B"] = to_device(o["hA"], gpu), to_device(o["hB"], gpu)
+        o["B0"] = o["B"].clone()
+        o["want"] = _expected(o)
+    cfgs, tuned = autotune.tune_batch(ops, max_impl=3, min_bytes=0, reps=1)
+    assert tuned >= 1 and len(cfgs) == len(ops)
+    for o in ops:  # the tuner ran the ops on its own buffers; B is untouched
+        assert same_bits(from_device(o["B"], o["hB"]), o["hB"])
+    batch = rt.Batch([dict(o, config=c) for o, c in zip(ops, cfgs)])
+    batch.execute()
+    torch.cuda.synchronize()
+    for i, o in enumerate(ops):
+        assert same_bits(from_device(o["B"], o["hB"]), o["want"]), (i, cfgs[i])
+    stream = torch.cuda.Stream()
+    for replay in range(2):
+        for o in ops:
+            o["B"].copy_(o["B0"])
+        torch.cuda.synchronize()
+        batch.graph_launch(stream)
+        stream.synchronize()
+        for i, o in enumerate(ops):
+            assert same_bits(from_device(o["B"], o["hB"]), o["want"]), (replay, i)
+
+
 def test_batch_rejects_unsupported_pairs(gpu):
     import torch
     from ttc_amd import runtime as rt

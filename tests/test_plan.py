@@ -211,3 +211,23 @@ def test_too_many_partial_dims_is_unsupported():
     with pytest.raises(rt.TTCError) as e:
         rt.Plan(S, S, [2, 1, 0], [10, 10, 10], config=dict(tile=[3, 3, 3], vec_a=1, vec_b=1))
     assert e.value.status == rt.ERROR_UNSUPPORTED
+
+
+def test_autotune_candidates_cover_families_orders_and_waves():
+    """The search space: the planner's choice first, both kernel families,
+    tile numberings, and fewer work-groups per CU; every candidate plans."""
+    from ttc_amd import autotune
+    perm, size = [2, 0, 1], [64, 48, 80]
+    cands = autotune.candidates(S, S, perm, size, beta_is_zero=False, max_impl=12)
+    assert cands[0] == {} and 2 <= len(cands) <= 12
+    assert {c["family"] for c in cands[1:]} == {1, 2}
+    assert any(c.get("tile_order") == 1 for c in cands) and any(c.get("wg_per_cu") for c in cands)
+    for c in cands:
+        rt.Plan(S, S, perm, size, beta_is_zero=False, config=c or None).info()
+    assert len(autotune.candidates(S, S, perm, size, max_impl=1)) == 1
+
+
+def test_batch_op_bytes():
+    from ttc_amd import autotune
+    assert autotune.op_bytes(dict(type_a=S, perm=[1, 0], size=[10, 20], beta=0.0)) == 200 * 8
+    assert autotune.op_bytes(dict(type_a=D, type_b=S, perm=[1, 0], size=[10, 20], beta=1.0)) == 200 * 16

@@ -6,7 +6,7 @@ survivor, keep the fastest (ttc/GPUtransposeGenerator.py:176-230, 328-349;
 ttc/ttc.py:871-1022) — is kept, with the search space of these kernels:
 kernel family (row x column or box tiles), tile shape (through run-length
 targets on A's and B's side), vector width per side, tile numbering,
-tile-walk order and non-temporal stores.  Nothing is compiled per candidate: every candidate is a launch of a
+tile-walk order, work-groups per CU and non-temporal stores.  Nothing is compiled per candidate: every candidate is a launch of a
 prebuilt kernel with different arguments, timed with HIP events.
 """
 import math
@@ -93,6 +93,15 @@ def candidates(type_a, type_b, perm, size, lda=None, ldb=None, beta_is_zero=Fals
             c = dict(t)
             c.update(l)
             out.append(c)
+    # waves per CU: the best tile with fewer resident work-groups than the
+    # occupancy limit (the default), where the budget leaves room
+    if ranked:
+        for w in (2, 3):
+            if len(out) < max_impl:
+                c = dict(ranked[0])
+                c.update(launch[0])
+                c["wg_per_cu"] = w
+                out.append(c)
     return out[:max(1, max_impl)]
 
 
@@ -164,3 +173,36 @@ def tune(type_a, type_b, perm, size, lda=None, ldb=None, alpha=1.0, beta=0.0, ma
             progress(i + 1, len(cands))
     results.sort(key=lambda r: r["ms"])
     return results
+
+
+def op_bytes(op):
+    """Algorithmic bytes of one batch op (dict as for runtime.Batch)."""
+    n = int(math.prod(op["size"]))
+    ta, tb = op["type_a"], op.get("type_b", op["type_a"])
+    bz = op.get("beta_is_zero", op.get("beta", 0.0) == 0)
+    return n * (rt.DTYPE_BYTES[ta] + rt.DTYPE_BYTES[tb] * (1 if bz else 2))
+
+
+def tune_batch(ops, max_impl=4, min_bytes=1 << 20, reps=3):
+    """Per-op configurations for a grouped batch (the `config` of each op
+    dict for runtime.Batch).  Every op of at least `min_bytes` algorithmic
+    bytes is tuned on its own shape, each distinct problem once; smaller
+    ops, whose time is launch and tail latency rather than tile shape, keep
+    the planner's choice (None).  Returns (configs, number of problems
+    tuned)."""
+    memo = {}
+    out = []
+    for op in ops:
+        if op_bytes(op) < min_bytes:
+            out.append(None)
+            continue
+        ta, tb = op["type_a"], op.get("type_b", op["type_a"])
+        bz = op.get("beta_is_zero", op.get("beta", 0.0) == 0)
+        key = (ta, tb, tuple(op["perm"]), tuple(op["size"]), tuple(op.get("lda") or ()),
+               tuple(op.get("ldb") or ()), bz)
+        if key not in memo:
+            res = tune(ta, tb, op["perm"], op["size"], op.get("lda"), op.get("ldb"), op.get("alpha", 1.0),
+                       0.0 if bz else 1.0, max_impl, reps)
+            memo[key] = res[0]["config"] if res and res[0]["config"] else None
+        out.append(memo[key])
+    return out, len(memo)

@@ -3,11 +3,14 @@
   python -m ttc_amd.profiling stats DIR [-o OUT.md]
       per-kernel launches, total and average duration from a
       `rocprofv3 --kernel-trace --stats` output directory.
+      With --kernel K --last N, also the durations of the last N dispatches
+      of kernel K from *kernel_trace.csv (the region that `bench.py` times
+      for its roofline line comes last among that kernel's launches).
   python -m ttc_amd.profiling counters DIR [-o OUT.md]
       per-kernel average of every counter of a `rocprofv3 --pmc` run
       (for example SQ_LDS_BANK_CONFLICT against SQ_ACTIVE_INST_LDS).
   python -m ttc_amd.profiling traffic --fetch DIR --write DIR
-         --calib-fetch DIR --calib-write DIR --calib-bytes N [-o OUT]
+         --calib-fetch DIR --calib-write DIR --calib-bytes N [--last N] [-o OUT]
       HBM bytes per launch of every kernel from two `rocprofv3 --pmc` runs
       (FETCH_SIZE, then WRITE_SIZE; KiB per dispatch), each scaled by what
       the same counter reads for a plain copy of N bytes (`bench.py
@@ -60,14 +63,34 @@ def kernel_stats(directory):
     return rows
 
 
+def last_dispatches(directory, kernel, n):
+    """(avg duration ns, min, max, span ns / n) of the last n dispatches of
+    `kernel` in *kernel_trace.csv; span runs from the first start to the
+    last end, so span / n includes the gaps between launches."""
+    rows = []
+    with open(_one(directory, "*kernel_trace.csv")) as f:
+        for r in csv.DictReader(f):
+            if short_name(r["Kernel_Name"]) == kernel:
+                rows.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"])))
+    rows.sort()
+    rows = rows[-n:]
+    if not rows:
+        raise ValueError("no dispatch of %s" % kernel)
+    d = [e - b for b, e in rows]
+    return sum(d) / len(d), min(d), max(d), (rows[-1][1] - rows[0][0]) / len(rows), len(rows)
+
+
 def counter_per_kernel(directory, counter):
     """{kernel: [value per dispatch]} of one counter from *counter_collection.csv."""
-    out = {}
+    rows = []
     with open(_one(directory, "*counter_collection.csv")) as f:
         for r in csv.DictReader(f):
-            if r["Counter_Name"] != counter:
-                continue
-            out.setdefault(short_name(r["Kernel_Name"]), []).append(float(r["Counter_Value"]))
+            if r["Counter_Name"] == counter:
+                rows.append((int(r["Dispatch_Id"]), short_name(r["Kernel_Name"]), float(r["Counter_Value"])))
+    rows.sort()  # dispatch order
+    out = {}
+    for _, k, v in rows:
+        out.setdefault(k, []).append(v)
     return out
 
 
@@ -78,7 +101,7 @@ def _copy_kernel(per_kernel):
     return max(names, key=lambda k: len(per_kernel[k]))
 
 
-def traffic(fetch_dir, write_dir, calib_fetch_dir, calib_write_dir, calib_bytes):
+def traffic(fetch_dir, write_dir, calib_fetch_dir, calib_write_dir, calib_bytes, last=0):
     cf = counter_per_kernel(calib_fetch_dir, "FETCH_SIZE")
     cw = counter_per_kernel(calib_write_dir, "WRITE_SIZE")
     kf, kw = _copy_kernel(cf), _copy_kernel(cw)
@@ -88,6 +111,9 @@ def traffic(fetch_dir, write_dir, calib_fetch_dir, calib_write_dir, calib_bytes)
     w_scale = calib_bytes / (write_kib * 1024.0)
     fetch = counter_per_kernel(fetch_dir, "FETCH_SIZE")
     write = counter_per_kernel(write_dir, "WRITE_SIZE")
+    if last:  # each kernel's last dispatches only
+        fetch = {k: v[-last:] for k, v in fetch.items()}
+        write = {k: v[-last:] for k, v in write.items()}
     kernels = {}
     for k in sorted(set(fetch) & set(write)):
         if "ttc::" not in k:
@@ -109,6 +135,8 @@ def main(argv=None):
     s.add_argument("dir")
     s.add_argument("-o", "--out")
     s.add_argument("--command", default="", help="the command that produced DIR (recorded in the summary)")
+    s.add_argument("--kernel", help="kernel whose last --last dispatches are summarised")
+    s.add_argument("--last", type=int, default=0)
     c = sub.add_parser("counters")
     c.add_argument("dir")
     c.add_argument("-o", "--out")
@@ -117,6 +145,7 @@ def main(argv=None):
     for a in ("--fetch", "--write", "--calib-fetch", "--calib-write"):
         t.add_argument(a, required=True)
     t.add_argument("--calib-bytes", type=int, required=True)
+    t.add_argument("--last", type=int, default=0, help="use only each kernel's last N dispatches")
     t.add_argument("-o", "--out", required=True, help="output path without extension")
     t.add_argument("--command", default="", help="the commands that produced the directories")
     args = ap.parse_args(argv)
@@ -127,6 +156,11 @@ def main(argv=None):
                  "| kernel | launches | total ms | avg us | % |", "|---|---:|---:|---:|---:|"]
         for name, calls, tot, avg in rows:
             lines.append("| `%s` | %d | %.3f | %.2f | %.1f |" % (name, calls, tot / 1e6, avg / 1e3, 100 * tot / total))
+        if args.kernel and args.last:
+            avg, lo, hi, span, n = last_dispatches(args.dir, args.kernel, args.last)
+            lines += ["", "Last %d dispatches of `%s`: average duration %.2f us (min %.2f, max %.2f); "
+                      "first start to last end / %d = %.2f us per launch, gaps included." % (
+                          n, args.kernel, avg / 1e3, lo / 1e3, hi / 1e3, n, span / 1e3)]
         text = "\n".join(lines) + "\n"
         if args.out:
             with open(args.out, "w") as f:
@@ -156,7 +190,8 @@ def main(argv=None):
                 f.write(text)
         sys.stdout.write(text)
         return 0
-    res = traffic(args.fetch, args.write, args.calib_fetch, args.calib_write, args.calib_bytes)
+    res = traffic(args.fetch, args.write, args.calib_fetch, args.calib_write, args.calib_bytes, args.last)
+    res["last_dispatches"] = args.last or None
     if args.command:
         res["command"] = args.command
     with open(args.out + ".json", "w") as f:
@@ -168,6 +203,7 @@ def main(argv=None):
     lines += ["Calibration: a plain copy of %d bytes (`%s`) reads FETCH_SIZE = %.1f KiB and WRITE_SIZE = %.1f KiB;"
               " scale factors %.4f (fetch) and %.4f (write)." % (c["copy_bytes"], c["copy_kernel"], c["fetch_kib"],
                                                                  c["write_kib"], c["fetch_scale"], c["write_scale"]),
+              "", ("Each kernel's last %d dispatches." % args.last) if args.last else "All dispatches.",
               "", "| kernel | launches | fetch MB/launch | write MB/launch | traffic MB/launch |",
               "|---|---:|---:|---:|---:|"]
     for k, v in sorted(res["kernels"].items(), key=lambda kv: -kv[1]["traffic_bytes"] * kv[1]["launches"]):
