@@ -317,6 +317,68 @@ def run_batch(torch, rt, wl, dev, stream, replays, tune=0):
             "grouped_GBps": round(info["bytes"] / (t_group * 1e-3) / 1e9, 1)}
 
 
+def run_workloads(torch, rt, wl, dev, stream, reps=5):
+    """Throughput of the mixed-precision / complex workload (perm (3,1,0,2),
+    size (256,192,128,96), alpha 1.3, beta 0.7) and of the fp8
+    cast-transposes (2-D 8192x14336 perm (1,0); 4-D (128,4096,32,8) perm
+    (0,2,1,3)) with the planner's configurations.  `reps` launches between
+    one pair of events; the fp8 launches alternate between two buffer pairs
+    so that the working set exceeds the last-level cache.  GB/s counts
+    algorithmic bytes (fp8: N*s_A + N; the current-scaling amax pass re-reads
+    A and is not counted)."""
+    import math
+
+    def timed(fn):
+        fn(0)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for r in range(reps):
+            fn(r)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    out = {}
+    c3 = wl.mixed_c3()
+    n = math.prod(c3["size"])
+    types = {"sb": (rt.S, rt.BF16, torch.float32, torch.bfloat16), "ds": (rt.D, rt.S, torch.float64, torch.float32),
+             "c": (rt.C, rt.C, torch.complex64, torch.complex64), "z": (rt.Z, rt.Z, torch.complex128, torch.complex128)}
+    for code in c3["dtypes"]:
+        ta, tb, da, db = types[code]
+        A = torch.randn(n, device=dev, dtype=da)
+        B = torch.randn(n, device=dev, dtype=db)
+        plan = rt.Plan(ta, tb, c3["perm"], c3["size"], beta_is_zero=False)
+        ms = timed(lambda r: plan.execute(A, B, c3["alpha"], c3["beta"], stream=stream))
+        nb = plan.info()["bytes_algorithmic"]
+        out["mixed_" + code] = {"ms": round(ms, 4), "GBps": round(nb / (ms * 1e-3) / 1e9, 1),
+                                "frac": round(nb / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        del A, B, plan
+        torch.cuda.empty_cache()
+    for shape in wl.fp8_c4():
+        n = math.prod(shape["size"])
+        for ta, da, an in ((rt.S, torch.float32, "f32"), (rt.BF16, torch.bfloat16, "bf16")):
+            As = [torch.randn(n, device=dev, dtype=da) for _ in range(2)]
+            for tb, bn in ((rt.E4M3, "e4m3"), (rt.E5M2, "e5m2")):
+                Bs = [torch.empty(n, dtype=torch.uint8, device=dev) for _ in range(2)]
+                amax = torch.zeros(1, dtype=torch.float32, device=dev)
+                sinv = torch.zeros(1, dtype=torch.float32, device=dev)
+                plan = rt.Plan(ta, tb, shape["perm"], shape["size"], beta_is_zero=True)
+                nb = n * (rt.DTYPE_BYTES[ta] + 1)
+                res = {}
+                for mode, mname in ((rt.SCALE_DELAYED, "delayed"), (rt.SCALE_CURRENT, "current")):
+                    ms = timed(lambda r: plan.execute_fp8(As[r % 2], Bs[r % 2], 1.0, scale=1.0, amax_out=amax,
+                                                          scale_inv_out=sinv, mode=mode, stream=stream))
+                    res[mname + "_ms"] = round(ms, 4)
+                    res[mname + "_GBps"] = round(nb / (ms * 1e-3) / 1e9, 1)
+                res["frac_delayed"] = round(res["delayed_GBps"] / HBM_PEAK_GBS, 4)
+                out["fp8_%s_%s_%s" % (an, bn, "x".join(str(x) for x in shape["size"]))] = res
+                del Bs, plan
+            del As
+            torch.cuda.empty_cache()
+    return out
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed counter summary."""
     try:
@@ -350,6 +412,8 @@ def main():
     ap.add_argument("--batch", dest="batch", action="store_true", default=True)
     ap.add_argument("--no-batch", dest="batch", action="store_false")
     ap.add_argument("--batch-replays", type=int, default=1000)
+    ap.add_argument("--no-workloads", dest="workloads", action="store_false", default=True,
+                    help="skip the mixed-precision / complex and fp8 cast-transpose throughput leg")
     ap.add_argument("--calibrate-copy", action="store_true", help="run only the plain copy used to scale counters")
     ap.add_argument("--tune", type=int, default=12,
                     help="autotune each op over at most N candidates before timing, as TTC generates the fastest "
@@ -435,6 +499,11 @@ def main():
             out["batched"] = run_batch(torch, rt, wl, dev, stream, args.batch_replays, min(args.tune, 4))
         except Exception as e:  # reported, not hidden
             out["batched"] = {"error": str(e)}
+    if args.workloads:
+        try:
+            out["workloads"] = run_workloads(torch, rt, wl, dev, stream)
+        except Exception as e:  # reported, not hidden
+            out["workloads"] = {"error": str(e)}
     out["cpu_baseline"] = None if args.no_cpu else cpu_baseline(args.cpu_seconds)
     print(json.dumps(out), flush=True)
 

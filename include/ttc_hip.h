@@ -74,8 +74,9 @@ typedef struct {
   int vec_b;             /* elements per lane vector on the B (write) side; 0 = widest legal */
   int wg_per_cu;         /* resident persistent work-groups per CU; 0 = occupancy limit */
   int nt_store;          /* non-temporal B stores (beta_is_zero only): 0 = auto (off), 1 = on, 2 = off */
-  int grid_order;        /* 0 = auto (3), 1 = strided tile walk, 2 = contiguous tile ranges,
-                            3 = strided walk with the work-groups of one XCD on consecutive tiles */
+  int grid_order;        /* 0 = auto (4), 1 = strided tile walk, 2 = contiguous tile ranges,
+                            3 = strided walk with the work-groups of one XCD on consecutive tiles,
+                            4 = as 3 within one contiguous share of the tiles per XCD */
   int run_a;             /* heuristic target run length on A's side, elements (0 = default) */
   int run_b;             /* heuristic target run length on B's side, elements (0 = default) */
   int family;            /* kernel family: 0 = auto, 1 = box tile, 2 = row x column tile */

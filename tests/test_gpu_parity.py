@@ -125,7 +125,7 @@ def test_every_vector_width_and_mode(gpu):
         for va in vec_widths(rt.DTYPE_BYTES[tA]):
             for vb in vec_widths(rt.DTYPE_BYTES[tB]):
                 for nt in (1, 2):
-                    for order in (1, 2, 3):
+                    for order in (1, 2, 3, 4):
                         for fam in (1, 2):
                             cfg = dict(vec_a=va, vec_b=vb, nt_store=nt, grid_order=order, wg_per_cu=1,
                                        family=fam)

@@ -73,7 +73,7 @@ def candidates(type_a, type_b, perm, size, lda=None, ldb=None, beta_is_zero=Fals
     ranked = heads + [c for c in ranked if all(c is not h for h in heads)]
     # launch variants: tile numbering (row x column tiles), tile walk, store mode
     launch = []
-    for order in (3, 1):
+    for order in (4, 3, 1):
         for tord in (2, 1):
             for nt in ((2, 1) if beta_is_zero and nt_stores else (0,)):
                 launch.append({"grid_order": order, "tile_order": tord, "nt_store": nt})

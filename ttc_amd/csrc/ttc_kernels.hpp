@@ -490,7 +490,18 @@ constexpr uint32_t kXcds = 8;
 
 __device__ __forceinline__ TileRange tile_range(uint32_t ntiles, uint32_t wg, uint32_t nwg, int order) {
   TileRange r;
-  if (order == 0 || order == 2) {
+  if (order == 3 && nwg % kXcds == 0) {
+    // XCD x (work-groups wg % kXcds == x) owns a contiguous 1/kXcds share of
+    // the tiles and its work-groups stride through it together: consecutive
+    // tiles still meet in one L2, and the last, partial round is spread over
+    // every XCD instead of landing on the first ones.
+    const uint32_t x = wg % kXcds, j = wg / kXcds, per = nwg / kXcds;
+    const uint32_t q = ntiles / kXcds, rem = ntiles % kXcds;
+    const uint32_t lo = x * q + (x < rem ? x : rem);
+    r.first = lo + j; r.end = lo + q + (x < rem ? 1u : 0u); r.step = per;
+    return r;
+  }
+  if (order == 0 || order == 2 || order == 3) {
     if (order == 2 && nwg % kXcds == 0) wg = (wg % kXcds) * (nwg / kXcds) + wg / kXcds;
     r.first = wg; r.end = ntiles; r.step = nwg;
   } else {

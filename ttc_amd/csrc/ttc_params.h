@@ -71,7 +71,7 @@ struct TileParams {
   int32_t tsplit0, tsplit1;    // t[split] (0 when none)
   int32_t lds_stride_b;        // ls[perm[0]]: LDS distance of consecutive B-order elements
   int32_t lds_elems;           // LDS elements used by one tile (with padding)
-  int32_t grid_order;          // 0 strided walk, 1 contiguous ranges
+  int32_t grid_order;          // tile_range order: 0 strided, 1 contiguous, 2 XCD-grouped, 3 XCD shares
 };
 
 // Row x column tile (ttc_rc.hpp): the tile's dimensions split into an A

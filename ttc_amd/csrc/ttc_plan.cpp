@@ -365,12 +365,14 @@ void choose_tile(const Geometry& g, int ta, int tb, int va, int vb, int run_a, i
 
 // ------------------------------------------------------------ launch params
 // ttc_config_t.grid_order -> the kernels' tile_range order: strided walk (0),
-// contiguous ranges (1), strided walk with XCD-grouped work-groups (2).
+// contiguous ranges (1), strided walk with XCD-grouped work-groups (2), one
+// contiguous share of the tiles per XCD walked by its work-groups (3).
 int kernel_grid_order(int cfg) {
   switch (cfg) {
     case 1: return 0;
     case 2: return 1;
-    default: return 2;
+    case 3: return 2;
+    default: return 3;
   }
 }
 
@@ -759,7 +761,7 @@ extern "C" ttc_status_t ttc_plan_create(ttc_plan_t* plan, ttc_dtype_t type_a, tt
   ttc_config_t c;
   std::memset(&c, 0, sizeof(c));
   if (cfg) c = *cfg;
-  if (c.nt_store < 0 || c.nt_store > 2 || c.grid_order < 0 || c.grid_order > 3 || c.wg_per_cu < 0 || c.vec_a < 0 ||
+  if (c.nt_store < 0 || c.nt_store > 2 || c.grid_order < 0 || c.grid_order > 4 || c.wg_per_cu < 0 || c.vec_a < 0 ||
       c.vec_b < 0 || c.family < 0 || c.family > 2 || c.tile_order < 0 || c.tile_order > 2)
     return TTC_ERROR_INVALID_VALUE;
   ttc_plan_s* pl = new (std::nothrow) ttc_plan_s();
@@ -845,7 +847,7 @@ extern "C" ttc_status_t ttc_plan_get_info(ttc_plan_t pl, ttc_plan_info_t* info) 
   info->lds_bytes = pl->lds_bytes;
   info->lds_pad = pl->lds_pad;
   info->nt_store = pl->nt_store;
-  info->grid_order = pl->p.grid_order == 0 ? 1 : (pl->p.grid_order == 1 ? 2 : 3);  // config numbering
+  info->grid_order = pl->p.grid_order + 1;  // config numbering
   info->wg_per_cu = pl->wg_per_cu;
   info->elements = g.elements;
   const int sa = dtype_bytes(pl->type_a), sb = dtype_bytes(pl->type_b);
