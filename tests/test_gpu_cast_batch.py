@@ -73,6 +73,32 @@ def test_fp8_cast_transpose_delayed_and_current(gpu, tA, tB):
             assert same_bits(got, want), (tA, tB, perm, size, alpha, "current", first_diff(got, want))
 
 
+@pytest.mark.parametrize("tB", [E4M3, E5M2])
+def test_fp8_conversion_every_binary32_input(gpu, tB):
+    """Every binary32 bit pattern (NaN, Inf, subnormals and both zeros
+    included) through the fp8 cast-transpose with alpha = scale = 1, in
+    chunks of 2^28, against torch's cast after clamping to the format's
+    range (round to nearest even, NaN -> sign | 0x7f): the saturating
+    conversion the oracle restates, bit for bit."""
+    import torch
+    from ttc_amd import runtime as rt
+    lim, tdt = (448.0, torch.float8_e4m3fn) if tB == E4M3 else (57344.0, torch.float8_e5m2)
+    side = 1 << 14
+    plan = rt.Plan(S, tB, [1, 0], [side, side], beta_is_zero=True)
+    B = torch.empty(side * side, dtype=torch.uint8, device=gpu)
+    for c in range(16):
+        u = (torch.arange(side * side, dtype=torch.int64, device=gpu) + (c << 28)).to(torch.int32)
+        x = u.view(torch.float32)
+        plan.execute_fp8(x, B, 1.0, scale=1.0)
+        want = torch.clamp(x, -lim, lim).to(tdt).view(torch.uint8)
+        torch.cuda.synchronize()
+        got2d, want2d = B.view(side, side), want.view(side, side).t()
+        if not torch.equal(got2d, want2d):
+            bad = (got2d != want2d).nonzero()[:8].cpu().tolist()
+            raise AssertionError("chunk %d: first mismatches at %s" % (c, bad))
+        del u, x, want
+
+
 def test_fp8_padded_and_misaligned(gpu):
     import torch
     from ttc_amd import runtime as rt

@@ -35,7 +35,8 @@ HIP_FLAGS = [
 ]
 CXX_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-Wall", "-I" + INCLUDE, "-I" + CSRC]
 
-HEADERS = ["ttc_kernels.hpp", "ttc_launch.hpp", "ttc_params.h", "ttc_plan.h", "ttc_runtime.h", "gen/inst_all.h"]
+HEADERS = ["ttc_kernels.hpp", "ttc_rc.hpp", "ttc_launch.hpp", "ttc_params.h", "ttc_plan.h", "ttc_runtime.h",
+           "gen/inst_all.h"]
 
 
 def _sources():

@@ -66,39 +66,23 @@ __device__ __forceinline__ f16s to_f16(float x) {
   return r;
 }
 
-// binary32 -> OCP fp8 e4m3fn, round to nearest even, saturating to +-448,
-// NaN -> 0x7f.  Normal range: round the binary32 mantissa to 3 bits in the
-// integer domain and rebias the exponent (127 -> 7); subnormal range
-// (|x| < 2^-6): the value is m * 2^-9 with m = rne(|x| * 2^9) (exact scaling).
-__device__ __forceinline__ uint8_t to_e4m3(float x) {
-  const uint32_t u = __float_as_uint(x);
-  const uint32_t sign = (u >> 24) & 0x80u;
-  const uint32_t a = u & 0x7fffffffu;
-  if (a > 0x7f800000u) return (uint8_t)(sign | 0x7fu);
-  if (a >= 0x43e00000u) return (uint8_t)(sign | 0x7eu);  // >= 448 (incl. Inf): saturate
-  if (a >= 0x3c800000u) {                                  // >= 2^-6: normal
-    uint32_t r = (a + 0x7ffffu + ((a >> 20) & 1u)) >> 20;
-    r -= (120u << 3);
-    return (uint8_t)(sign | r);
-  }
-  const float m = __builtin_rintf(__uint_as_float(a) * 512.0f);
-  return (uint8_t)(sign | (uint32_t)m);
-}
-// binary32 -> OCP fp8 e5m2, round to nearest even, saturating to +-57344,
-// NaN -> 0x7f.  Exponent rebias 127 -> 15; subnormal quantum 2^-16.
-__device__ __forceinline__ uint8_t to_e5m2(float x) {
-  const uint32_t u = __float_as_uint(x);
-  const uint32_t sign = (u >> 24) & 0x80u;
-  const uint32_t a = u & 0x7fffffffu;
-  if (a > 0x7f800000u) return (uint8_t)(sign | 0x7fu);
-  if (a >= 0x47600000u) return (uint8_t)(sign | 0x7bu);  // >= 57344 (incl. Inf): saturate
-  if (a >= 0x38800000u) {                                  // >= 2^-14: normal
-    uint32_t r = (a + 0xfffffu + ((a >> 21) & 1u)) >> 21;
-    r -= (112u << 2);
-    return (uint8_t)(sign | r);
-  }
-  const float m = __builtin_rintf(__uint_as_float(a) * 65536.0f);
-  return (uint8_t)(sign | (uint32_t)m);
+// Two binary32 values -> OCP fp8 (e4m3fn, or e5m2 with E5), round to
+// nearest even with gradual underflow, saturating to +-448 (e5m2: +-57344,
+// Inf included), NaN -> sign | 0x7f; byte 0 from a, byte 1 from b.  The
+// hardware conversion rounds; the clamp in front of it gives the saturation,
+// and NaN is fixed up after it.  Over all 2^32 binary32 inputs this equals
+// rounding the mantissa to 3 (e5m2: 2) bits in the integer domain with the
+// exponent rebiased, and m * 2^-9 (e5m2: 2^-16) with m = rne(|x| * 2^9
+// (2^16)) below the normal range.
+template <bool E5> __device__ __forceinline__ uint32_t fp8_pair(float a, float b) {
+  constexpr float lim = E5 ? 57344.0f : 448.0f;
+  const float ca = __builtin_amdgcn_fmed3f(a, -lim, lim), cb = __builtin_amdgcn_fmed3f(b, -lim, lim);
+  const uint32_t r = E5 ? (uint32_t)__builtin_amdgcn_cvt_pk_bf8_f32(ca, cb, 0, false)
+                        : (uint32_t)__builtin_amdgcn_cvt_pk_fp8_f32(ca, cb, 0, false);
+  uint32_t qa = r & 0xffu, qb = (r >> 8) & 0xffu;
+  if (a != a) qa = ((__float_as_uint(a) >> 24) & 0x80u) | 0x7fu;
+  if (b != b) qb = ((__float_as_uint(b) >> 24) & 0x80u) | 0x7fu;
+  return qa | (qb << 8);
 }
 
 template <class T> __device__ __forceinline__ T from_f32(float x);
@@ -152,15 +136,21 @@ using OpZZ = OpCplx<OpDD, c64, c64, c64>;
 using OpCZ = OpCplx<OpSD, c32, c64, c32>;
 using OpZC = OpCplx<OpDS, c64, c32, c64>;
 
-// fp8 outputs: q = sat_rne(fl32(fl32(al*a) * scale)).
+// fp8 outputs: q = sat_rne(fl32(fl32(al*a) * scale)), converted in pairs.
 template <class TA_, class TB_> struct OpF8 {
   using TA = TA_; using TB = TB_; using L = float; using AL = float; using BE = float;
+  static constexpr bool E5 = __is_same(TB, e5m2s);
   __device__ static L prod(AL al, TA a) { return al * to_f32(a); }
   __device__ static TB outs(L x, float scale) {
     TB r;
     const float y = x * scale;
-    if constexpr (__is_same(TB, e4m3s)) r.u = to_e4m3(y); else r.u = to_e5m2(y);
+    r.u = (uint8_t)fp8_pair<E5>(y, y);
     return r;
+  }
+  __device__ static void outs2(L x0, L x1, float scale, TB& o0, TB& o1) {
+    const uint32_t q = fp8_pair<E5>(x0 * scale, x1 * scale);
+    o0.u = (uint8_t)q;
+    o1.u = (uint8_t)(q >> 8);
   }
 };
 
@@ -404,8 +394,12 @@ struct TileWorker {
         const int li = lds_b(s);
         TB o[VB];
         if constexpr (FP8) {
+          if constexpr (VB == 1) {
+            o[0] = OP::outs(lds[li], scale);
+          } else {
 #pragma unroll
-          for (int v = 0; v < VB; ++v) o[v] = OP::outs(lds[li + v * S], scale);
+            for (int v = 0; v < VB; v += 2) OP::outs2(lds[li + v * S], lds[li + (v + 1) * S], scale, o[v], o[v + 1]);
+          }
         } else if constexpr (BETA) {
           TB ob[VB];
           vload<TB, VB>(ob, reinterpret_cast<const TB*>(b + offB[s]));
