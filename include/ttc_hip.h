@@ -79,7 +79,8 @@ typedef struct {
                             4 = as 3 within one contiguous share of the tiles per XCD */
   int run_a;             /* heuristic target run length on A's side, elements (0 = default) */
   int run_b;             /* heuristic target run length on B's side, elements (0 = default) */
-  int family;            /* kernel family: 0 = auto, 1 = box tile, 2 = row x column tile */
+  int family;            /* kernel family: 0 = auto, 1 = box tile, 2 = row x column tile,
+                            3 = row copy (perm[0] == 0 after fusion only) */
   int tile_order;        /* tile numbering (row x column tiles): 0 = auto (B order), 1 = A order, 2 = B order */
 } ttc_config_t;
 
@@ -106,7 +107,7 @@ typedef struct {
   int type_a, type_b;
   int beta_is_zero;
   int run_a, run_b;              /* contiguous element runs of a full tile in A and B */
-  int family;                    /* 1 = box tile kernel, 2 = row x column tile kernel */
+  int family;                    /* 1 = box tile kernel, 2 = row x column tile kernel, 3 = row copy */
   int rows, cols;                /* row x column tile: row and column counts (0 for box tiles) */
   int tile_order;                /* 1 = A order, 2 = B order */
 } ttc_plan_info_t;

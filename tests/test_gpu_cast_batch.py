@@ -27,7 +27,7 @@ def _heavy_input(tA, n, seed):
 
 
 FP8_SHAPES = [([1, 0], [512, 448]), ([0, 2, 1, 3], [128, 64, 32, 8]), ([2, 0, 1], [33, 17, 40]),
-              ([1, 0], [1000, 37])]
+              ([1, 0], [1000, 37]), ([0, 2, 1], [7, 9, 5])]
 
 
 @pytest.mark.parametrize("tA", [S, BF16, F16])

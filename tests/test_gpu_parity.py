@@ -72,29 +72,34 @@ RANDOM_SHAPES = [
 
 def families(tA, tB, perm, size, lda=None, ldb=None):
     """Kernel families a shape can run on: the box tile always, the row x
-    column tile when A's and B's leading dimensions differ."""
+    column tile when A's and B's leading dimensions differ, the row copy
+    when they coincide (dimension 0 in place after fusion)."""
     from ttc_amd import runtime as rt
-    try:
-        rt.Plan(tA, tB, perm, size, lda, ldb, True, dict(family=2))
-        return [1, 2]
-    except rt.TTCError as e:
-        assert e.status == rt.ERROR_UNSUPPORTED
-        return [1]
+    out = [1]
+    for fam in (2, 3):
+        try:
+            rt.Plan(tA, tB, perm, size, lda, ldb, True, dict(family=fam))
+            out.append(fam)
+        except rt.TTCError as e:
+            assert e.status == rt.ERROR_UNSUPPORTED
+    assert len(out) == 2 or rt.fuse_indices(perm, size, lda, ldb)[0] == [0]
+    return out
 
 
 @pytest.mark.parametrize("code", ["s", "d", "c", "z", "sd", "ds", "cz", "zc", "b", "h", "sb", "sh", "bs", "hs"])
 def test_random_shapes(gpu, code):
     tA, tB = CODE[code]
-    n_rc = 0
+    n_rc = n_row = 0
     for i, (perm, size) in enumerate(RANDOM_SHAPES):
         for fam in families(tA, tB, perm, size):
             n_rc += fam == 2
+            n_row += fam == 3
             for beta in (0.0, 0.7):
                 order = 1 + (i + int(beta > 0)) % 2  # row x column tiles: A- and B-order numbering
                 info = check(gpu, tA, tB, perm, size, 1.3, beta, seed=100 + i, dist="outliers",
                              config=dict(family=fam, tile_order=order))
                 assert info["family"] == fam and info["tile_order"] == (order if fam == 2 else 1)
-    assert n_rc >= 9
+    assert n_rc >= 9 and n_row >= 4
 
 
 def test_padded_subtensor(gpu):
@@ -167,6 +172,27 @@ def test_explicit_row_column_tiles(gpu):
     assert n >= 40
 
 
+def test_row_copy_every_vector_width_and_mode(gpu):
+    """Row-copy family (dimension 0 in place): every instantiated width,
+    store mode and work-group count, dense and padded."""
+    from ttc_amd import runtime as rt
+    from ttc_amd.gen_instances import row_widths
+    for perm, size in (([0, 2, 1], [64, 48, 80]), ([0, 3, 1, 2], [16, 5, 7, 9]), ([0, 1], [8, 999])):
+        for code in ("s", "d", "b", "c", "z", "sb", "ds", "zc"):
+            tA, tB = CODE[code]
+            for v in row_widths(rt.DTYPE_BYTES[tA], rt.DTYPE_BYTES[tB]):
+                for nt in (1, 2):
+                    for wg in (0, 1):
+                        cfg = dict(vec_a=v, vec_b=v, nt_store=nt, wg_per_cu=wg, family=3)
+                        for beta in (0.0, 1.5):
+                            info = check(gpu, tA, tB, perm, size, 1.25, beta, config=cfg, seed=v * 10 + nt)
+                            assert info["family"] == 3 and info["vec_a"] == v
+    lda, ldb = [68, 50, 81], [68, 83, 49]
+    for code in ("s", "d", "sb"):
+        tA, tB = CODE[code]
+        check(gpu, tA, tB, [0, 2, 1], [64, 48, 80], 1.0, 1.0, lda, ldb, config=dict(family=3))
+
+
 def test_misaligned_pointers_use_scalar_variant(gpu):
     for off in (1, 3):
         for fam in (1, 2):
@@ -174,6 +200,9 @@ def test_misaligned_pointers_use_scalar_variant(gpu):
             check(gpu, S, S, [1, 0], [256, 192], 1.0, 1.0, offset_a=off, offset_b=0, config=cfg)
             check(gpu, S, S, [1, 0], [256, 192], 1.0, 0.0, offset_a=0, offset_b=off, config=cfg)
             check(gpu, BF16, BF16, [1, 0], [256, 192], 1.0, 0.0, offset_a=off, offset_b=off, config=cfg)
+        info = check(gpu, S, S, [0, 2, 1], [256, 6, 30], 1.0, 1.0, offset_a=off, offset_b=0)
+        assert info["family"] == 3 and info["vec_a"] == 4
+        check(gpu, BF16, S, [0, 2, 1], [256, 6, 30], 1.0, 0.0, offset_a=0, offset_b=off)
 
 
 def test_edge_shapes(gpu):

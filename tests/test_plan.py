@@ -106,13 +106,21 @@ def _check_plan(i, perm, size):
         V *= i["tile"][k]
         tiles *= -(-i["size"][k] // i["tile"][k]) if i["size"][k] else 0
     assert V == i["tile_volume"]
-    assert tiles == i["tiles"] or i["elements"] == 0
+    assert tiles == i["tiles"] or i["elements"] == 0 or i["family"] == 3
     assert i["tile"][0] % i["vec_a"] == 0 and i["tile"][i["perm"][0]] % i["vec_b"] == 0
     assert i["size"][0] % i["vec_a"] == 0 and i["size"][i["perm"][0]] % i["vec_b"] == 0
     def slots(vec, nbytes):  # ttc_params.h slots_for
         return min(16, max(4, 64 // (vec * nbytes)))
     def rc_elems(nbytes):  # ttc_params.h rc_elems
         return min(32, 128 // nbytes)
+    if i["family"] == 3:
+        # row copy: dimension 0 stays in place; one vector width for both
+        # sides, at most 16 bytes on each; no LDS
+        assert i["perm"][0] == 0 and i["vec_a"] == i["vec_b"]
+        assert i["tile"][0] == i["size"][0] and V == i["size"][0]
+        assert i["vec_a"] * max(rt.DTYPE_BYTES[i["type_a"]], rt.DTYPE_BYTES[i["type_b"]]) <= 16
+        assert i["lds_bytes"] == 0 and i["elements"] == int(np.prod(size))
+        return
     if i["family"] == 1:
         assert i["slots_a"] <= slots(i["vec_a"], rt.DTYPE_BYTES[i["type_a"]])
         assert i["slots_b"] <= slots(i["vec_b"], rt.DTYPE_BYTES[i["type_b"]])
@@ -231,3 +239,25 @@ def test_batch_op_bytes():
     from ttc_amd import autotune
     assert autotune.op_bytes(dict(type_a=S, perm=[1, 0], size=[10, 20], beta=0.0)) == 200 * 8
     assert autotune.op_bytes(dict(type_a=D, type_b=S, perm=[1, 0], size=[10, 20], beta=1.0)) == 200 * 16
+
+
+def test_row_copy_family_for_perm0():
+    """Dimension 0 in place after fusion -> row copy, with the widest vector
+    both sides allow; box tiles on request; never for other permutations."""
+    for ta, tb, want_v in ((S, S, 4), (D, D, 2), (S, rt.E4M3, 4), (rt.BF16, rt.E5M2, 8), (S, rt.BF16, 4),
+                           (rt.Z, rt.Z, 1)):
+        i = rt.Plan(ta, tb, [0, 2, 1, 3], [128, 4096, 32, 8], beta_is_zero=True).info()
+        assert i["family"] == 3 and i["vec_a"] == want_v == i["vec_b"], (ta, tb, i)
+        assert i["tiles"] == 4096 * 32 * 8
+    assert rt.Plan(S, S, [0, 2, 1], [6, 5, 7], beta_is_zero=False).info()["vec_a"] == 2
+    assert rt.Plan(S, S, [0, 2, 1], [7, 5, 3], beta_is_zero=False).info()["vec_a"] == 1
+    assert rt.Plan(S, S, [0, 1, 2], [7, 5, 3]).info()["family"] == 3  # identity: one row
+    assert rt.Plan(S, S, [0, 2, 1], [8, 5, 7], config={"family": 1}).info()["family"] == 1
+    with pytest.raises(rt.TTCError):
+        rt.Plan(S, S, [1, 0, 2], [8, 5, 7], config={"family": 3})
+    # explicit widths of the row copy: one width, from its own list
+    i = rt.Plan(S, rt.BF16, [0, 2, 1], [64, 48, 80], config={"family": 3, "vec_a": 4, "vec_b": 4}).info()
+    assert i["family"] == 3 and i["vec_a"] == i["vec_b"] == 4
+    for bad in ({"vec_a": 4, "vec_b": 2}, {"vec_a": 8}, {"vec_b": 16}):
+        with pytest.raises(rt.TTCError):
+            rt.Plan(S, rt.BF16, [0, 2, 1], [64, 48, 80], config=dict(bad, family=3))

@@ -4,7 +4,8 @@ TTC's method — enumerate blockings x loop orders, prune to about
 sqrt(maxImplementations) of each by cost models, compile and time every
 survivor, keep the fastest (ttc/GPUtransposeGenerator.py:176-230, 328-349;
 ttc/ttc.py:871-1022) — is kept, with the search space of these kernels:
-kernel family (row x column or box tiles), tile shape (through run-length
+kernel family (row x column tiles, box tiles, or row copy when dimension 0
+stays in place), tile shape (through run-length
 targets on A's and B's side), vector width per side, tile numbering,
 tile-walk order, work-groups per CU and non-temporal stores.  Nothing is compiled per candidate: every candidate is a launch of a
 prebuilt kernel with different arguments, timed with HIP events.
@@ -86,6 +87,8 @@ def candidates(type_a, type_b, perm, size, lda=None, ldb=None, beta_is_zero=Fals
     while len(ranked) * len(launch) > budget and len(launch) > 1:
         launch.pop()
     out = [{}]
+    if base["family"] == 3:  # row copy (perm[0] == 0): resident work-groups per CU
+        out += [{"family": 3, "wg_per_cu": w} for w in (4, 2)]
     for t in ranked:
         for l in launch:
             if t["family"] == 1 and l["tile_order"] != 2:

@@ -438,6 +438,15 @@ extern "C" ttc_status_t ttc_batch_create(ttc_batch_t* out, int nops, const ttc_b
       ttc_status_t st = ttc_plan_create(&pl, op.type_a, op.type_b, op.dim, op.perm, op.size,
                                         any_lda ? op.lda : nullptr, any_ldb ? op.ldb : nullptr,
                                         op.beta_is_zero, &cfg);
+      if (st == TTC_SUCCESS && pl->family == kFamilyRow) {
+        // grouped launches have no row-copy kind: box tiles for this op
+        ttc_plan_destroy(pl);
+        pl = nullptr;
+        cfg.family = kFamilyBox;
+        for (int k = 0; k < TTC_MAX_DIM; ++k) cfg.tile[k] = 0;
+        st = ttc_plan_create(&pl, op.type_a, op.type_b, op.dim, op.perm, op.size, any_lda ? op.lda : nullptr,
+                             any_ldb ? op.ldb : nullptr, op.beta_is_zero, &cfg);
+      }
       if (st != TTC_SUCCESS) {
         batch_free(b);
         return st;

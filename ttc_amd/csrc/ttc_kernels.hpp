@@ -27,6 +27,10 @@
 
 namespace ttc {
 
+// Kernel modes: beta != 0, beta == 0, beta == 0 with non-temporal stores,
+// fp8 cast-transpose.
+enum : int { kModeBeta = 0, kModeBz = 1, kModeBzNt = 2, kModeFp8 = 3 };
+
 // ----------------------------------------------------------------- types
 struct alignas(8) c32 { float re, im; };
 struct alignas(16) c64 { double re, im; };
@@ -60,6 +64,10 @@ __device__ __forceinline__ bf16s to_bf16(float x) {
 // binary32 -> binary16 with the hardware conversion (round to nearest even,
 // gradual underflow, overflow to Inf).
 __device__ __forceinline__ f16s to_f16(float x) {
+  // x is a rounded binary32 value: keep the conversion its own instruction,
+  // or the compiler may fold the multiply that produced x into a mixed
+  // multiply-convert (v_fma_mixlo_f16) that rounds only once.
+  asm volatile("" : "+v"(x));
   _Float16 h = (_Float16)x;
   f16s r;
   __builtin_memcpy(&r.u, &h, 2);

@@ -6,13 +6,12 @@
 
 #include "ttc_kernels.hpp"
 #include "ttc_rc.hpp"
+#include "ttc_row.hpp"
 
 namespace ttc {
 
-// mode of a transpose entry
-enum : int { kModeBeta = 0, kModeBz = 1, kModeBzNt = 2, kModeFp8 = 3 };
 // kernel family (matches ttc_config_t.family)
-enum : int { kFamBox = 1, kFamRC = 2 };
+enum : int { kFamBox = 1, kFamRC = 2, kFamRow = 3 };
 
 using LaunchFn = hipError_t (*)(const TileParams&, const void*, void*, double, double, unsigned, size_t, hipStream_t);
 using LaunchFp8Fn = hipError_t (*)(const TileParams&, const void*, void*, float, const float*, const float*, float,
@@ -21,12 +20,16 @@ using LaunchFp8Fn = hipError_t (*)(const TileParams&, const void*, void*, float,
 using LaunchRCFn = hipError_t (*)(const RCParams&, const void*, void*, double, double, unsigned, size_t,
                                   hipStream_t);
 
+using LaunchRowFn = hipError_t (*)(const RowParams&, const void*, void*, double, double, const float*, const float*,
+                                   float, float, float*, float*, unsigned, hipStream_t);
+
 struct KernelEntry {
   int ta, tb, va, vb, mode, family;
   LaunchFn launch;
   LaunchFp8Fn launch_fp8;
   LaunchRCFn launch_rc;
   const void* func;
+  LaunchRowFn launch_row;
 };
 
 template <class OP, int VA, int VB, bool BETA, bool NT>
@@ -70,6 +73,26 @@ KernelEntry make_rc_entry(int ta, int tb) {
   KernelEntry e{ta, tb, VA, VB, MODE, kFamRC, nullptr, nullptr, nullptr, nullptr};
   e.launch_rc = &launch_rc<OP, VA, VB, beta, nt>;
   e.func = reinterpret_cast<const void*>(&transpose_rc_kernel<OP, VA, VB, beta, nt>);
+  return e;
+}
+
+template <class OP, int V, int MODE>
+hipError_t launch_row(const RowParams& p, const void* A, void* B, double alpha, double beta, const float* scale_in,
+                      const float* amax_in, float fp8_max, float scale_host, float* amax_out, float* scale_inv_out,
+                      unsigned grid, hipStream_t s) {
+  const typename OP::TA* a = static_cast<const typename OP::TA*>(A);
+  typename OP::TB* b = static_cast<typename OP::TB*>(B);
+  RowParams pp = p;
+  void* args[] = {&pp, &a, &b, &alpha, &beta, &scale_in, &amax_in, &fp8_max, &scale_host, &amax_out, &scale_inv_out};
+  return hipLaunchKernel(reinterpret_cast<const void*>(&row_kernel<OP, V, MODE>), dim3(grid), dim3(kThreads), args,
+                         0, s);
+}
+
+template <class OP, int V, int MODE>
+KernelEntry make_row_entry(int ta, int tb) {
+  KernelEntry e{ta, tb, V, V, MODE, kFamRow, nullptr, nullptr, nullptr, nullptr, nullptr};
+  e.launch_row = &launch_row<OP, V, MODE>;
+  e.func = reinterpret_cast<const void*>(&row_kernel<OP, V, MODE>);
   return e;
 }
 

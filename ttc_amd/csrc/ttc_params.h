@@ -105,6 +105,22 @@ struct RCParams {
   int32_t lds_spare;       // RP * C: where lanes past a partial tile stage their (unused) values
 };
 
+// Row copy for perm[0] == 0, where A and B share the stride-1 dimension:
+// a row is one position of the other dimensions, the rows are walked in B
+// order, and each lane moves one vector of V elements of dimension 0.
+constexpr int kRowUnroll = 4;  // vectors in flight per lane in the row-copy kernel
+
+struct RowParams {
+  uint32_t total;         // rows * nv (< 2^31)
+  uint32_t nv;            // vectors per row: extent of dimension 0 / V
+  FastDiv dnv;
+  int32_t rank;           // row dimensions, in B order
+  int32_t pad_;
+  FastDiv dn[kMaxDim];    // their extents
+  int64_t sa[kMaxDim];    // A byte strides
+  int64_t sb[kMaxDim];    // B byte strides
+};
+
 // One row x column tile of a grouped launch: element offsets of its origin
 // in A and B; flags bit 0: last (partial) tile along the row group, bit 1:
 // along the column group.

@@ -633,6 +633,33 @@ ttc_status_t build_rc_params(const Geometry& g, const int* tile, int ta, int tb,
   return TTC_SUCCESS;
 }
 
+// Row copy: rows are the positions of dimensions 1..rank-1 in B order
+// (perm[1], perm[2], ...); a work item is one vector of v elements of
+// dimension 0.  Offsets are 64-bit in the kernel, so no span limit applies;
+// the vector count must stay below 2^31 for the 32-bit index walk.
+ttc_status_t build_row_params(const Geometry& g, int ta, int tb, int v, RowParams& p) {
+  std::memset(&p, 0, sizeof(p));
+  if (g.rank < 1 || g.perm[0] != 0 || v < 1 || g.size[0] % v) return TTC_ERROR_UNSUPPORTED;
+  if (g.elements == 0) return TTC_SUCCESS;  // total = 0: nothing is launched
+  const int szA = dtype_bytes(ta), szB = dtype_bytes(tb);
+  const long long nv = g.size[0] / v;
+  long long rows = 1;
+  p.rank = g.rank - 1;
+  for (int q = 1; q < g.rank; ++q) {
+    const int k = g.perm[q];
+    p.dn[q - 1] = make_fastdiv((uint32_t)g.size[k]);
+    p.sa[q - 1] = g.sa[k] * szA;
+    p.sb[q - 1] = g.sb[k] * szB;
+    rows *= g.size[k];
+  }
+  const long long total = rows * nv;
+  if (total >= (1ll << 31)) return TTC_ERROR_UNSUPPORTED;
+  p.total = (uint32_t)total;
+  p.nv = (uint32_t)nv;
+  p.dnv = make_fastdiv((uint32_t)nv);
+  return TTC_SUCCESS;
+}
+
 bool choose_rc_tile(const Geometry& g, int ta, int tb, int va, int vb, int run_a, int run_b, int* tile,
                     double* score) {
   if (g.elements == 0 || g.rank < 2 || g.perm[0] == 0) return false;
@@ -701,6 +728,34 @@ static ttc_status_t plan_fill(ttc_plan_s* pl, int va, int vb, const int* tile_in
     for (int k = 0; k < g.rank; ++k) explicit_tile |= tile_in[k] != 0;
   const bool fp8 = pl->type_b == TTC_E4M3 || pl->type_b == TTC_E5M2;
   int fam = family;
+  if (fam == kFamilyRow && g.perm[0] != 0) return TTC_ERROR_UNSUPPORTED;
+  if (fam == kFamilyRow || (fam == 0 && !explicit_tile && g.perm[0] == 0)) {
+    // A and B share the stride-1 dimension: rows move without LDS staging.
+    // One vector width for both sides: the widest instantiated one within
+    // both sides' widths (legal on both, since they are).
+    int v = 1;
+    for (int w : row_widths(dtype_bytes(pl->type_a), dtype_bytes(pl->type_b)))
+      if (w <= va && w <= vb && va % w == 0 && vb % w == 0) v = std::max(v, w);
+    RowParams rp;
+    const ttc_status_t st = build_row_params(g, pl->type_a, pl->type_b, v, rp);
+    if (st == TTC_SUCCESS) {
+      pl->va = pl->vb = v;
+      pl->family = kFamilyRow;
+      for (int k = 0; k < kMaxDim; ++k) pl->tile[k] = k == 0 ? g.size[0] : 1;
+      std::memset(&pl->p, 0, sizeof(pl->p));
+      std::memset(&pl->rc, 0, sizeof(pl->rc));
+      pl->row = rp;
+      const long long chunk = (long long)kThreads * kRowUnroll;
+      pl->p.ntiles = (uint32_t)((rp.total + chunk - 1) / chunk);
+      pl->p.V = g.size[0];
+      pl->lds_bytes = 0;
+      pl->lds_pad = 0;
+      pl->run_a = pl->run_b = g.size[0];
+      return TTC_SUCCESS;
+    }
+    if (fam == kFamilyRow) return st;
+    fam = 0;  // too many vectors for the row walk: tile kernels
+  }
   if (explicit_tile) {
     for (int k = 0; k < kMaxDim; ++k) tile[k] = k < g.rank ? tile_in[k] : 1;
     if (fam == 0) fam = kFamilyBox;
@@ -762,7 +817,7 @@ extern "C" ttc_status_t ttc_plan_create(ttc_plan_t* plan, ttc_dtype_t type_a, tt
   std::memset(&c, 0, sizeof(c));
   if (cfg) c = *cfg;
   if (c.nt_store < 0 || c.nt_store > 2 || c.grid_order < 0 || c.grid_order > 4 || c.wg_per_cu < 0 || c.vec_a < 0 ||
-      c.vec_b < 0 || c.family < 0 || c.family > 2 || c.tile_order < 0 || c.tile_order > 2)
+      c.vec_b < 0 || c.family < 0 || c.family > 3 || c.tile_order < 0 || c.tile_order > 2)
     return TTC_ERROR_INVALID_VALUE;
   ttc_plan_s* pl = new (std::nothrow) ttc_plan_s();
   if (!pl) return TTC_ERROR_ALLOC;
@@ -791,8 +846,27 @@ extern "C" ttc_status_t ttc_plan_create(ttc_plan_t* plan, ttc_dtype_t type_a, tt
   };
   int va = c.vec_a ? c.vec_a : widest_vec_a(pl->g, type_a);
   int vb = c.vec_b ? c.vec_b : widest_vec_b(pl->g, type_b);
-  if (!in_list(va, dtype_bytes(type_a)) || !in_list(vb, dtype_bytes(type_b)) || !vec_ok_a(pl->g, va) ||
-      !vec_ok_b(pl->g, vb)) {
+  if (c.family == kFamilyRow) {
+    // the row copy has one width for both sides (its own instantiated list)
+    auto in_row = [&](int w) {
+      for (int x : row_widths(dtype_bytes(type_a), dtype_bytes(type_b)))
+        if (x == w) return true;
+      return false;
+    };
+    if (c.vec_a && c.vec_b && c.vec_a != c.vec_b) {
+      delete pl;
+      return TTC_ERROR_INVALID_VALUE;
+    }
+    if (c.vec_a || c.vec_b) {
+      const int w = c.vec_a ? c.vec_a : c.vec_b;
+      if (!in_row(w) || !vec_ok_a(pl->g, w) || !vec_ok_b(pl->g, w)) {
+        delete pl;
+        return TTC_ERROR_INVALID_VALUE;
+      }
+      va = vb = w;
+    }
+  } else if (!in_list(va, dtype_bytes(type_a)) || !in_list(vb, dtype_bytes(type_b)) || !vec_ok_a(pl->g, va) ||
+             !vec_ok_b(pl->g, vb)) {
     delete pl;
     return TTC_ERROR_INVALID_VALUE;
   }
@@ -864,6 +938,10 @@ extern "C" ttc_status_t ttc_plan_get_info(ttc_plan_t pl, ttc_plan_info_t* info) 
     info->slots_b = rc_elems(sb) / pl->vb;
     info->rows = pl->rc.R;
     info->cols = pl->rc.C;
+  }
+  if (pl->family == kFamilyRow) {
+    info->slots_a = info->slots_b = kRowUnroll;
+    info->tiles = g.size[0] > 0 ? g.elements / g.size[0] : 0;  // rows
   }
   return TTC_SUCCESS;
 }

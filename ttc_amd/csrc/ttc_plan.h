@@ -20,6 +20,17 @@ inline std::vector<int> vec_widths(int bytes) {
   return w;
 }
 
+// Vector widths of the row-copy kernel: one width for both sides, at most
+// 16 bytes on each (1, 2 and the widest).  The kernel generator uses the
+// same rule.
+inline std::vector<int> row_widths(int bytes_a, int bytes_b) {
+  std::vector<int> w{1};
+  const int wide = 16 / (bytes_a > bytes_b ? bytes_a : bytes_b);
+  if (wide >= 2) w.push_back(2);
+  if (wide > 2) w.push_back(wide);
+  return w;
+}
+
 int dtype_bytes(int t);
 // Bytes of the LDS staging type (alpha * a in A's arithmetic) for a type pair.
 int lds_elem_bytes(int ta, int tb);
@@ -46,6 +57,7 @@ ttc_status_t make_geometry(int dim, const int* perm, const int* size, const int*
 // Kernel families (ttc_config_t.family).
 constexpr int kFamilyBox = 1;
 constexpr int kFamilyRC = 2;
+constexpr int kFamilyRow = 3;
 
 struct PlanChoice {
   int tile[kMaxDim];
@@ -84,6 +96,10 @@ ttc_status_t build_rc_params(const Geometry& g, const int* tile, int ta, int tb,
 bool choose_rc_tile(const Geometry& g, int ta, int tb, int va, int vb, int run_a, int run_b, int* tile,
                     double* score);
 
+// Row-copy family (perm[0] == 0): launch parameters for vectors of v
+// elements; TTC_ERROR_UNSUPPORTED when the shape does not fit the kernel.
+ttc_status_t build_row_params(const Geometry& g, int ta, int tb, int v, RowParams& p);
+
 }  // namespace ttc
 
 struct ttc_plan_s {
@@ -97,8 +113,9 @@ struct ttc_plan_s {
   ttc::TileParams p;
   int lds_bytes, lds_pad;
   int run_a, run_b;
-  int family;            // 1 box tile kernel (p), 2 row x column kernel (rc)
+  int family;            // 1 box tile kernel (p), 2 row x column kernel (rc), 3 row copy (row)
   int tile_order;        // 1 A order, 2 B order (row x column tiles only)
   ttc::RCParams rc;
+  ttc::RowParams row;
   ttc_plan_s* fallback;  // scalar-vector plan for misaligned pointers (may be null)
 };

@@ -3,6 +3,7 @@
 // that keeps the reference wrapper's semantics.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -84,6 +85,19 @@ unsigned grid_size(uint32_t ntiles, int wg_per_cu, const void* func, int lds_byt
   return (unsigned)(ntiles < g ? ntiles : g);
 }
 
+// Row-copy launches keep no LDS and stream with more resident work-groups:
+// 8 per CU unless the plan says otherwise (or occupancy allows fewer).
+static unsigned row_grid(const ttc_plan_s* use, int wg_per_cu, const void* func) {
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  const int occ = occupancy(func, 0, kThreads);
+  int wg = wg_per_cu > 0 ? wg_per_cu : std::min(occ, 8);
+  if (wg < 1) wg = 1;
+  const uint64_t g = (uint64_t)device_cus(dev) * (uint64_t)wg;
+  const uint64_t n = use->p.ntiles;
+  return (unsigned)(n < g ? n : g);
+}
+
 static bool aligned(const void* p, int bytes) { return ((uintptr_t)p % (uintptr_t)bytes) == 0; }
 
 // Picks the plan variant whose vector widths the pointers allow.
@@ -104,6 +118,13 @@ ttc_status_t launch_plan(const ttc_plan_s* pl, const void* A, void* B, double al
   const int mode = pl->beta_is_zero ? (pl->nt_store ? kModeBzNt : kModeBz) : kModeBeta;
   const KernelEntry* e = find_kernel(pl->type_a, pl->type_b, use->va, use->vb, mode, use->family);
   if (!e) return TTC_ERROR_UNSUPPORTED;
+  if (use->family == kFamRow) {
+    const unsigned grid = row_grid(use, pl->wg_per_cu, e->func);
+    return e->launch_row(use->row, A, B, alpha, beta, nullptr, nullptr, 0.0f, 1.0f, nullptr, nullptr, grid, s) ==
+                   hipSuccess
+               ? TTC_SUCCESS
+               : TTC_ERROR_HIP;
+  }
   const unsigned grid = grid_size(use->p.ntiles, pl->wg_per_cu, e->func, use->lds_bytes,
                                   use->family == kFamRC ? kRcThreads : kThreads);
   const hipError_t err = use->family == kFamRC
@@ -183,7 +204,8 @@ extern "C" ttc_status_t ttc_execute_fp8(ttc_plan_t pl, const void* A, void* B, d
   if (!A || !B) return TTC_ERROR_INVALID_VALUE;
   const ttc_plan_s* use = variant_for(pl, A, B);
   if (!use) return TTC_ERROR_INVALID_VALUE;
-  const KernelEntry* e = find_kernel(pl->type_a, pl->type_b, use->va, use->vb, kModeFp8, kFamBox);
+  const KernelEntry* e = find_kernel(pl->type_a, pl->type_b, use->va, use->vb, kModeFp8,
+                                     use->family == kFamRow ? kFamRow : kFamBox);
   if (!e) return TTC_ERROR_UNSUPPORTED;
   const float fp8_max = pl->type_b == TTC_E4M3 ? 448.0f : 57344.0f;
   const float al = (float)alpha;
@@ -201,6 +223,13 @@ extern "C" ttc_status_t ttc_execute_fp8(ttc_plan_t pl, const void* A, void* B, d
     amax_in = d_amax_out;
     amax_out = nullptr;
     d_scale_in = nullptr;
+  }
+  if (use->family == kFamRow) {
+    const unsigned grid = row_grid(use, pl->wg_per_cu, e->func);
+    return e->launch_row(use->row, A, B, alpha, 0.0, d_scale_in, amax_in, fp8_max, scale, amax_out, d_scale_inv_out,
+                         grid, s) == hipSuccess
+               ? TTC_SUCCESS
+               : TTC_ERROR_HIP;
   }
   const unsigned grid = grid_size(use->p.ntiles, pl->wg_per_cu, e->func, use->lds_bytes, kThreads);
   return e->launch_fp8(use->p, A, B, al, d_scale_in, amax_in, fp8_max, scale, amax_out, d_scale_inv_out, grid,
