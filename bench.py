@@ -317,13 +317,15 @@ def run_batch(torch, rt, wl, dev, stream, replays, tune=0):
             "grouped_GBps": round(info["bytes"] / (t_group * 1e-3) / 1e9, 1)}
 
 
-def run_workloads(torch, rt, wl, dev, stream, reps=5):
+def run_workloads(torch, rt, wl, dev, stream, reps=5, tune=0):
     """Throughput of the mixed-precision / complex workload (perm (3,1,0,2),
     size (256,192,128,96), alpha 1.3, beta 0.7) and of the fp8
     cast-transposes (2-D 8192x14336 perm (1,0); 4-D (128,4096,32,8) perm
     (0,2,1,3)) with the planner's configurations.  `reps` launches between
     one pair of events; the fp8 launches alternate between two buffer pairs
-    so that the working set exceeds the last-level cache.  GB/s counts
+    so that the working set exceeds the last-level cache.  tune > 0: each
+    fp8 cast takes the autotuner's configuration (at most `tune`
+    candidates), with the planner's timed beside it.  GB/s counts
     algorithmic bytes (fp8: N*s_A + N; the current-scaling amax pass re-reads
     A and is not counted)."""
     import math
@@ -366,6 +368,16 @@ def run_workloads(torch, rt, wl, dev, stream, reps=5):
                 plan = rt.Plan(ta, tb, shape["perm"], shape["size"], beta_is_zero=True)
                 nb = n * (rt.DTYPE_BYTES[ta] + 1)
                 res = {}
+                if tune > 0:  # the planner's choice, then the autotuner's (delayed scaling)
+                    from ttc_amd import autotune
+                    res["planner_delayed_GBps"] = round(nb / (timed(lambda r: plan.execute_fp8(
+                        As[r % 2], Bs[r % 2], 1.0, scale=1.0, stream=stream)) * 1e-3) / 1e9, 1)
+                    best = autotune.tune(ta, tb, shape["perm"], shape["size"], alpha=1.0, beta=0.0, max_impl=tune,
+                                         reps=3)
+                    if best and best[0]["config"]:
+                        plan = rt.Plan(ta, tb, shape["perm"], shape["size"], beta_is_zero=True,
+                                       config=best[0]["config"])
+                    res["plan"] = {k: plan.info()[k] for k in ("family", "tile", "vec_a", "vec_b")}
                 for mode, mname in ((rt.SCALE_DELAYED, "delayed"), (rt.SCALE_CURRENT, "current")):
                     ms = timed(lambda r: plan.execute_fp8(As[r % 2], Bs[r % 2], 1.0, scale=1.0, amax_out=amax,
                                                           scale_inv_out=sinv, mode=mode, stream=stream))
@@ -501,7 +513,7 @@ def main():
             out["batched"] = {"error": str(e)}
     if args.workloads:
         try:
-            out["workloads"] = run_workloads(torch, rt, wl, dev, stream)
+            out["workloads"] = run_workloads(torch, rt, wl, dev, stream, tune=min(args.tune, 8))
         except Exception as e:  # reported, not hidden
             out["workloads"] = {"error": str(e)}
     out["cpu_baseline"] = None if args.no_cpu else cpu_baseline(args.cpu_seconds)

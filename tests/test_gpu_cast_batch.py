@@ -36,41 +36,44 @@ def test_fp8_cast_transpose_delayed_and_current(gpu, tA, tB):
     import torch
     from ttc_amd import runtime as rt
     for i, (perm, size) in enumerate(FP8_SHAPES):
-        n = int(np.prod(size))
-        A = _heavy_input(tA, n, 40 + i)
-        dA = to_device(A, gpu)
-        dB = torch.zeros(n, dtype=torch.uint8, device=gpu)
-        amax = torch.full((1,), -1.0, dtype=torch.float32, device=gpu)
-        sinv = torch.zeros(1, dtype=torch.float32, device=gpu)
-        plan = rt.Plan(tA, tB, perm, size, beta_is_zero=True)
-        for alpha in (1.0, 0.5):
-            # delayed scaling with a host scale, then with a device scale
-            for scale in (1.0, 3.5):
+        for cfg in (None, dict(family=1)):  # the planner's family (row x column or row copy), box tiles
+            n = int(np.prod(size))
+            A = _heavy_input(tA, n, 40 + i)
+            dA = to_device(A, gpu)
+            dB = torch.zeros(n, dtype=torch.uint8, device=gpu)
+            amax = torch.full((1,), -1.0, dtype=torch.float32, device=gpu)
+            sinv = torch.zeros(1, dtype=torch.float32, device=gpu)
+            plan = rt.Plan(tA, tB, perm, size, beta_is_zero=True, config=cfg)
+            if cfg:
+                assert plan.info()["family"] == 1
+            for alpha in (1.0, 0.5):
+                # delayed scaling with a host scale, then with a device scale
+                for scale in (1.0, 3.5):
+                    want = np.zeros(n, np.uint8)
+                    want_amax = orc.cast_fp8(tA, tB, A, want, alpha, scale, perm, size)
+                    plan.execute_fp8(dA, dB, alpha, scale=scale, amax_out=amax, scale_inv_out=sinv)
+                    torch.cuda.synchronize()
+                    got = dB.cpu().numpy()
+                    assert same_bits(got, want), (tA, tB, perm, size, alpha, scale, first_diff(got, want))
+                    assert float(amax.item()) == want_amax
+                    assert float(sinv.item()) == float(np.float32(1.0) / np.float32(scale))
+                    dscale = torch.tensor([scale], dtype=torch.float32, device=gpu)
+                    dB.zero_()
+                    plan.execute_fp8(dA, dB, alpha, scale=0.0, scale_in=dscale)
+                    torch.cuda.synchronize()
+                    assert same_bits(dB.cpu().numpy(), want)
+                # current scaling: amax pass, then scale = fp8_max / amax
+                ref = np.zeros(n, np.uint8)
+                a_max = orc.cast_fp8(tA, tB, A, ref, alpha, 1.0, perm, size)
+                scale = np.float32(FP8_MAX[tB]) / np.float32(a_max)
                 want = np.zeros(n, np.uint8)
-                want_amax = orc.cast_fp8(tA, tB, A, want, alpha, scale, perm, size)
-                plan.execute_fp8(dA, dB, alpha, scale=scale, amax_out=amax, scale_inv_out=sinv)
+                orc.cast_fp8(tA, tB, A, want, alpha, float(scale), perm, size)
+                plan.execute_fp8(dA, dB, alpha, amax_out=amax, scale_inv_out=sinv, mode=rt.SCALE_CURRENT)
                 torch.cuda.synchronize()
+                assert float(amax.item()) == a_max
+                assert float(sinv.item()) == float(np.float32(1.0) / scale)
                 got = dB.cpu().numpy()
-                assert same_bits(got, want), (tA, tB, perm, size, alpha, scale, first_diff(got, want))
-                assert float(amax.item()) == want_amax
-                assert float(sinv.item()) == float(np.float32(1.0) / np.float32(scale))
-                dscale = torch.tensor([scale], dtype=torch.float32, device=gpu)
-                dB.zero_()
-                plan.execute_fp8(dA, dB, alpha, scale=0.0, scale_in=dscale)
-                torch.cuda.synchronize()
-                assert same_bits(dB.cpu().numpy(), want)
-            # current scaling: amax pass, then scale = fp8_max / amax
-            ref = np.zeros(n, np.uint8)
-            a_max = orc.cast_fp8(tA, tB, A, ref, alpha, 1.0, perm, size)
-            scale = np.float32(FP8_MAX[tB]) / np.float32(a_max)
-            want = np.zeros(n, np.uint8)
-            orc.cast_fp8(tA, tB, A, want, alpha, float(scale), perm, size)
-            plan.execute_fp8(dA, dB, alpha, amax_out=amax, scale_inv_out=sinv, mode=rt.SCALE_CURRENT)
-            torch.cuda.synchronize()
-            assert float(amax.item()) == a_max
-            assert float(sinv.item()) == float(np.float32(1.0) / scale)
-            got = dB.cpu().numpy()
-            assert same_bits(got, want), (tA, tB, perm, size, alpha, "current", first_diff(got, want))
+                assert same_bits(got, want), (tA, tB, perm, size, alpha, "current", first_diff(got, want))
 
 
 @pytest.mark.parametrize("tB", [E4M3, E5M2])

@@ -43,3 +43,29 @@ def test_generate_transposition_measures_caches_and_emits(gpu, tmp_path, monkeyp
     assert os.path.exists(os.path.join(str(tmp_path), "ttc_transpositions", name + ".hip"))
     name2, gibs2 = ttc.generateTransposition(args())  # found in the database: no new measurement
     assert name2 == name and gibs2 == pytest.approx(gibs)
+
+
+def test_autotune_fp8_candidates_agree(gpu):
+    """The search also covers fp8 cast-transposes (timed with delayed
+    scaling); every candidate reproduces the oracle's bits and amax."""
+    import numpy as np
+    import torch
+    from common import make_case, same_bits, to_device
+    from oracle import oracle as orc
+    from ttc_amd import autotune
+    from ttc_amd import runtime as rt
+    perm, size = [1, 0], [512, 448]
+    results = autotune.tune(S, rt.E4M3, perm, size, alpha=1.0, beta=0.0, max_impl=8, reps=2)
+    assert len(results) >= 3 and {r["info"]["family"] for r in results} >= {1, 2}
+    A, _ = make_case(S, S, perm, size, seed=5, dist="heavy")
+    want = np.zeros(A.size, np.uint8)
+    want_amax = orc.cast_fp8(S, rt.E4M3, A, want, 0.75, 2.0, perm, size)
+    dA = to_device(A, gpu)
+    for r in results:
+        plan = rt.Plan(S, rt.E4M3, perm, size, beta_is_zero=True, config=r["config"] or None)
+        dB = torch.zeros(A.size, dtype=torch.uint8, device=gpu)
+        amax = torch.zeros(1, dtype=torch.float32, device=gpu)
+        plan.execute_fp8(dA, dB, 0.75, scale=2.0, amax_out=amax)
+        torch.cuda.synchronize()
+        assert same_bits(dB.cpu().numpy(), want), r["config"]
+        assert float(amax.item()) == want_amax

@@ -18,7 +18,8 @@ from . import runtime as rt
 def tile_score(info, size_a, size_b):
     """Same model as the planner (csrc/ttc_plan.cpp tile_score and
     choose_rc_tile): fraction of useful tile slots x b/(b+64) per side for
-    contiguous runs of b bytes x V/(V+256), times u/(0.3+0.7u) for the
+    contiguous runs of b bytes, each side weighted by its share of the bytes
+    (exponent 2 s/(s_A + s_B)), x V/(V+256), times u/(0.3+0.7u) for the
     row x column kernel, whose tiles fill a fraction u of its slots."""
     n = info["elements"]
     if n == 0:
@@ -26,7 +27,8 @@ def tile_score(info, size_a, size_b):
     eff = n / float(info["tiles"] * info["tile_volume"])
     ba, bb = info["run_a"] * size_a, info["run_b"] * size_b
     v = info["tile_volume"]
-    sc = eff * (ba / (ba + 64.0)) * (bb / (bb + 64.0)) * (v / (v + 256.0))
+    wa, wb = 2.0 * size_a / (size_a + size_b), 2.0 * size_b / (size_a + size_b)
+    sc = eff * (ba / (ba + 64.0)) ** wa * (bb / (bb + 64.0)) ** wb * (v / (v + 256.0))
     if info["family"] == 2:
         cap = info["threads"] * min(32, 128 // max(size_a, size_b))
         u = v / float(cap)
@@ -65,6 +67,14 @@ def candidates(type_a, type_b, perm, size, lda=None, ldb=None, beta_is_zero=Fals
                         tiles[key] = (tile_score(info, sa, sb),
                                       {"tile": info["tile"], "vec_a": va, "vec_b": vb, "family": fam})
     ranked = [c for _, c in sorted(tiles.values(), key=lambda x: -x[0])]
+    # distinct tile shapes first (each with its best-scoring widths), then the
+    # other widths of shapes already listed, so that pruning keeps shapes
+    seen, first, rest = set(), [], []
+    for c in ranked:
+        key = (c["family"], tuple(c["tile"]))
+        (rest if key in seen else first).append(c)
+        seen.add(key)
+    ranked = first + rest
     # the best tile of each family first, so that pruning never drops a family
     heads = []
     for fam in (2, 1):
@@ -119,14 +129,20 @@ def measure(plan, pairs, alpha, beta, reps=3, stream=None):
     to the kernel.)"""
     import torch
     s = stream or torch.cuda.current_stream()  # launches and events on the same stream
+    if plan.info()["type_b"] in (rt.E4M3, rt.E5M2):  # fp8 cast-transpose, delayed scaling
+        def run(A, B):
+            plan.execute_fp8(A, B, alpha, scale=1.0, stream=s)
+    else:
+        def run(A, B):
+            plan.execute(A, B, alpha, beta, stream=s)
     for A, B in pairs:
-        plan.execute(A, B, alpha, beta, stream=s)
+        run(A, B)
     rounds = []
     for _ in range(reps):
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record(s)
         for A, B in pairs:
-            plan.execute(A, B, alpha, beta, stream=s)
+            run(A, B)
         e1.record(s)
         e1.synchronize()
         rounds.append(e0.elapsed_time(e1) / len(pairs))

@@ -20,6 +20,9 @@ using LaunchFp8Fn = hipError_t (*)(const TileParams&, const void*, void*, float,
 using LaunchRCFn = hipError_t (*)(const RCParams&, const void*, void*, double, double, unsigned, size_t,
                                   hipStream_t);
 
+using LaunchFp8RCFn = hipError_t (*)(const RCParams&, const void*, void*, float, const float*, const float*, float,
+                                     float, float*, float*, unsigned, size_t, hipStream_t);
+
 using LaunchRowFn = hipError_t (*)(const RowParams&, const void*, void*, double, double, const float*, const float*,
                                    float, float, float*, float*, unsigned, hipStream_t);
 
@@ -30,6 +33,7 @@ struct KernelEntry {
   LaunchRCFn launch_rc;
   const void* func;
   LaunchRowFn launch_row;
+  LaunchFp8RCFn launch_fp8_rc;
 };
 
 template <class OP, int VA, int VB, bool BETA, bool NT>
@@ -66,13 +70,30 @@ hipError_t launch_rc(const RCParams& p, const void* A, void* B, double alpha, do
                          dim3(kRcThreads), args, lds, s);
 }
 
+template <class OP, int VA, int VB>
+hipError_t launch_fp8_rc(const RCParams& p, const void* A, void* B, float alpha, const float* scale_in,
+                         const float* amax_in, float fp8_max, float scale_host, float* amax_out, float* scale_inv_out,
+                         unsigned grid, size_t lds, hipStream_t s) {
+  const typename OP::TA* a = static_cast<const typename OP::TA*>(A);
+  typename OP::TB* b = static_cast<typename OP::TB*>(B);
+  RCParams pp = p;
+  void* args[] = {&pp, &a, &b, &alpha, &scale_in, &amax_in, &fp8_max, &scale_host, &amax_out, &scale_inv_out};
+  return hipLaunchKernel(reinterpret_cast<const void*>(&cast_fp8_rc_kernel<OP, VA, VB>), dim3(grid),
+                         dim3(kRcThreads), args, lds, s);
+}
+
 template <class OP, int VA, int VB, int MODE>
 KernelEntry make_rc_entry(int ta, int tb) {
-  constexpr bool beta = MODE == kModeBeta;
-  constexpr bool nt = MODE == kModeBzNt;
   KernelEntry e{ta, tb, VA, VB, MODE, kFamRC, nullptr, nullptr, nullptr, nullptr};
-  e.launch_rc = &launch_rc<OP, VA, VB, beta, nt>;
-  e.func = reinterpret_cast<const void*>(&transpose_rc_kernel<OP, VA, VB, beta, nt>);
+  if constexpr (MODE == kModeFp8) {
+    e.launch_fp8_rc = &launch_fp8_rc<OP, VA, VB>;
+    e.func = reinterpret_cast<const void*>(&cast_fp8_rc_kernel<OP, VA, VB>);
+  } else {
+    constexpr bool beta = MODE == kModeBeta;
+    constexpr bool nt = MODE == kModeBzNt;
+    e.launch_rc = &launch_rc<OP, VA, VB, beta, nt>;
+    e.func = reinterpret_cast<const void*>(&transpose_rc_kernel<OP, VA, VB, beta, nt>);
+  }
   return e;
 }
 

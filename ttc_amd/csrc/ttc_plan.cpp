@@ -337,7 +337,10 @@ double tile_score(const Geometry& g, const int* tile, int szA, int szB) {
   tile_runs(g, tile, ra, rb);
   const double eff = (double)g.elements / ((double)ntiles * (double)V);
   const double ba = (double)ra * szA, bb = (double)rb * szB;
-  return eff * (ba / (ba + 64.0)) * (bb / (bb + 64.0)) * ((double)V / ((double)V + 256.0));
+  // each side's run efficiency b/(b+64), weighted by its share of the bytes
+  // moved (a weighted geometric mean; equal element sizes give the product)
+  const double wa = 2.0 * szA / (szA + szB), wb = 2.0 * szB / (szA + szB);
+  return eff * std::pow(ba / (ba + 64.0), wa) * std::pow(bb / (bb + 64.0), wb) * ((double)V / ((double)V + 256.0));
 }
 
 void choose_tile(const Geometry& g, int ta, int tb, int va, int vb, int run_a, int run_b, int* tile) {
@@ -726,7 +729,6 @@ static ttc_status_t plan_fill(ttc_plan_s* pl, int va, int vb, const int* tile_in
   bool explicit_tile = false;
   if (tile_in)
     for (int k = 0; k < g.rank; ++k) explicit_tile |= tile_in[k] != 0;
-  const bool fp8 = pl->type_b == TTC_E4M3 || pl->type_b == TTC_E5M2;
   int fam = family;
   if (fam == kFamilyRow && g.perm[0] != 0) return TTC_ERROR_UNSUPPORTED;
   if (fam == kFamilyRow || (fam == 0 && !explicit_tile && g.perm[0] == 0)) {
@@ -762,7 +764,7 @@ static ttc_status_t plan_fill(ttc_plan_s* pl, int va, int vb, const int* tile_in
   } else {
     int rc_tile[kMaxDim];
     double rc_score = -1.0;
-    const bool rc_ok = !fp8 && fam != kFamilyBox &&
+    const bool rc_ok = fam != kFamilyBox &&
                        choose_rc_tile(g, pl->type_a, pl->type_b, va, vb, run_a, run_b, rc_tile, &rc_score);
     if (fam == kFamilyRC && !rc_ok) return TTC_ERROR_UNSUPPORTED;
     if (fam != kFamilyRC) choose_tile(g, pl->type_a, pl->type_b, va, vb, run_a, run_b, tile);
@@ -779,7 +781,6 @@ static ttc_status_t plan_fill(ttc_plan_s* pl, int va, int vb, const int* tile_in
     if (fam == kFamilyRC)
       for (int k = 0; k < kMaxDim; ++k) tile[k] = rc_tile[k];
   }
-  if (fam == kFamilyRC && fp8) return TTC_ERROR_UNSUPPORTED;
   pl->va = va;
   pl->vb = vb;
   pl->family = fam;

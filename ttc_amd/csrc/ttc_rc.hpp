@@ -40,6 +40,9 @@ struct RCWorker {
   static constexpr int NIB = rc_elems((int)sizeof(TB)) / VB;  // write vectors per thread per tile
   static constexpr uint32_t SA = kRcThreads * VA;               // read-walk step between a thread's slots
   static constexpr uint32_t SB = kRcThreads * VB;               // write-walk step
+  static constexpr bool FP8 = is_fp8<OP>::value;                // fp8 cast: beta carries the scale
+
+  mutable float amax = 0.0f;  // fp8 cast: max |alpha * a| over this thread's elements
 
   L* data;              // LDS tile
   const int32_t* rowA;  // LDS tables, byte offsets
@@ -224,7 +227,11 @@ struct RCWorker {
       const bool ok = (int32_t)(it * SA) < c.limA;
       const uint32_t a = ok ? li : (uint32_t)p.lds_spare;
 #pragma unroll
-      for (int v = 0; v < VA; ++v) data[a + v] = OP::prod(alpha, va[it][v]);
+      for (int v = 0; v < VA; ++v) {
+        const L x = OP::prod(alpha, va[it][v]);
+        if constexpr (FP8) amax = fmaxf(amax, fabsf(x));  // lanes past the tile loaded zeros
+        data[a + v] = x;
+      }
       i += c.adi;
       li += dli;
       if (i >= c.rows) {
@@ -276,8 +283,17 @@ struct RCWorker {
       TB o[VB];
 #pragma unroll
       for (int v = 0; v < VB; ++v) {
-        if constexpr (BETA) o[v] = OP::outb(data[li + v * RP], beta, ob[it][v]);
-        else o[v] = OP::out(data[li + v * RP]);
+        if constexpr (FP8) {
+          if constexpr (VB == 1) {
+            o[0] = OP::outs(data[li], beta);
+          } else if (v % 2 == 0) {
+            OP::outs2(data[li + v * RP], data[li + (v + 1) * RP], beta, o[v], o[v + 1]);
+          }
+        } else if constexpr (BETA) {
+          o[v] = OP::outb(data[li + v * RP], beta, ob[it][v]);
+        } else {
+          o[v] = OP::out(data[li + v * RP]);
+        }
       }
       bstore<TB, VB, NT>(r, off, o);
     };
@@ -356,6 +372,33 @@ __global__ void __launch_bounds__(kRcThreads) transpose_rc_kernel(const RCParams
   // tile coordinates derived from them) in scalar registers.
   w.run(p, kp, A, B, uniform(r.first), uniform(r.end), uniform(r.step), (typename OP::AL)alpha,
         (typename OP::BE)beta);
+}
+
+// fp8 cast-transpose on row x column tiles: q = sat_rne(fl32(fl32(alpha*a)
+// * scale)), amax = max |fl32(alpha*a)| (one atomic per work-group).  The
+// scale comes from *scale_in (delayed scaling), from fp8_max / *amax_in
+// (current scaling, amax_in from a previous pass), or from the host value.
+template <class OP, int VA, int VB>
+__global__ void __launch_bounds__(kRcThreads)
+    cast_fp8_rc_kernel(const RCParams p, const typename OP::TA* __restrict__ A, typename OP::TB* __restrict__ B,
+                       float alpha, const float* __restrict__ scale_in, const float* __restrict__ amax_in,
+                       float fp8_max, float scale_host, float* __restrict__ amax_out,
+                       float* __restrict__ scale_inv_out) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  KParams* kp = (KParams*)__builtin_amdgcn_kernarg_segment_ptr();
+  float scale = scale_host;
+  if (scale_in) {
+    scale = *scale_in;
+  } else if (amax_in) {
+    const float m = *amax_in;
+    scale = (m > 0.0f && m <= 3.402823466e38f) ? fp8_max / m : 1.0f;
+  }
+  if (scale_inv_out && blockIdx.x == 0 && threadIdx.x == 0) *scale_inv_out = 1.0f / scale;
+  RCWorker<OP, VA, VB, false, false> w;
+  w.setup(p, lds_raw, threadIdx.x);
+  const TileRange r = tile_range(p.ntiles, blockIdx.x, gridDim.x, p.grid_order);
+  w.run(p, kp, A, B, uniform(r.first), uniform(r.end), uniform(r.step), alpha, scale);
+  if (amax_out) block_amax_commit(w.amax, amax_out);
 }
 
 }  // namespace ttc

@@ -204,8 +204,7 @@ extern "C" ttc_status_t ttc_execute_fp8(ttc_plan_t pl, const void* A, void* B, d
   if (!A || !B) return TTC_ERROR_INVALID_VALUE;
   const ttc_plan_s* use = variant_for(pl, A, B);
   if (!use) return TTC_ERROR_INVALID_VALUE;
-  const KernelEntry* e = find_kernel(pl->type_a, pl->type_b, use->va, use->vb, kModeFp8,
-                                     use->family == kFamRow ? kFamRow : kFamBox);
+  const KernelEntry* e = find_kernel(pl->type_a, pl->type_b, use->va, use->vb, kModeFp8, use->family);
   if (!e) return TTC_ERROR_UNSUPPORTED;
   const float fp8_max = pl->type_b == TTC_E4M3 ? 448.0f : 57344.0f;
   const float al = (float)alpha;
@@ -228,6 +227,13 @@ extern "C" ttc_status_t ttc_execute_fp8(ttc_plan_t pl, const void* A, void* B, d
     const unsigned grid = row_grid(use, pl->wg_per_cu, e->func);
     return e->launch_row(use->row, A, B, alpha, 0.0, d_scale_in, amax_in, fp8_max, scale, amax_out, d_scale_inv_out,
                          grid, s) == hipSuccess
+               ? TTC_SUCCESS
+               : TTC_ERROR_HIP;
+  }
+  if (use->family == kFamRC) {
+    const unsigned grid = grid_size(use->p.ntiles, pl->wg_per_cu, e->func, use->lds_bytes, kRcThreads);
+    return e->launch_fp8_rc(use->rc, A, B, al, d_scale_in, amax_in, fp8_max, scale, amax_out, d_scale_inv_out, grid,
+                            (size_t)use->lds_bytes, s) == hipSuccess
                ? TTC_SUCCESS
                : TTC_ERROR_HIP;
   }
