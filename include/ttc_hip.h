@@ -81,7 +81,8 @@ typedef struct {
   int run_b;             /* heuristic target run length on B's side, elements (0 = default) */
   int family;            /* kernel family: 0 = auto, 1 = box tile, 2 = row x column tile,
                             3 = row copy (perm[0] == 0 after fusion only) */
-  int tile_order;        /* tile numbering (row x column tiles): 0 = auto (B order), 1 = A order, 2 = B order */
+  int tile_order;        /* tile numbering (row x column tiles): 0 = auto (B order), 1 = A order, 2 = B order,
+                            3 / 4 = B order in groups of 8 / 16 tiles along A's leading tiled dimension */
 } ttc_config_t;
 
 /* What a plan resolved to (fused geometry, tile, launch shape). */

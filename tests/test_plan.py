@@ -229,7 +229,8 @@ def test_autotune_candidates_cover_families_orders_and_waves():
     cands = autotune.candidates(S, S, perm, size, beta_is_zero=False, max_impl=12)
     assert cands[0] == {} and 2 <= len(cands) <= 12
     assert {c["family"] for c in cands[1:]} == {1, 2}
-    assert any(c.get("tile_order") == 1 for c in cands) and any(c.get("wg_per_cu") for c in cands)
+    assert any(c.get("tile_order") == 3 for c in cands) and any(c.get("wg_per_cu") for c in cands)
+    assert any(c.get("tile_order") == 1 for c in autotune.candidates(S, S, perm, size, max_impl=40))
     for c in cands:
         rt.Plan(S, S, perm, size, beta_is_zero=False, config=c or None).info()
     assert len(autotune.candidates(S, S, perm, size, max_impl=1)) == 1

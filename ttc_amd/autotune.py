@@ -85,7 +85,7 @@ def candidates(type_a, type_b, perm, size, lda=None, ldb=None, beta_is_zero=Fals
     # launch variants: tile numbering (row x column tiles), tile walk, store mode
     launch = []
     for order in (4, 3, 1):
-        for tord in (2, 1):
+        for tord in (2, 3, 1):  # B order, B order grouped by 8 along A's rows, A order
             for nt in ((2, 1) if beta_is_zero and nt_stores else (0,)):
                 launch.append({"grid_order": order, "tile_order": tord, "nt_store": nt})
     # sqrt pruning of both axes, as the reference combines its best blockings

@@ -99,7 +99,12 @@ struct RCParams {
   FastDiv gdiv[kMaxDim];   // division by the number of tiles along the dimension
   int64_t tsa[kMaxDim], tsb[kMaxDim];  // element offset of the next tile along it, in A and B
   int32_t splitA, splitB;  // grid positions of the row group's / column group's partial dimension (-1 none)
-  uint32_t ntiles;
+  // Grouped walk: one dimension is split into an inner position 0 of
+  // `group` tiles and an outer position `gouter` (-1: no split); `gcount`
+  // is its number of real tiles, ids past it are empty padding.
+  int32_t gouter;
+  uint32_t group, gcount;
+  uint32_t ntiles;         // grid size (with the padding of a grouped walk)
   int32_t grid_order;
   int32_t lds_data_elems;  // tile data region in LDS elements (RP * C, a spare vector slot, 16-byte rounding)
   int32_t lds_spare;       // RP * C: where lanes past a partial tile stage their (unused) values

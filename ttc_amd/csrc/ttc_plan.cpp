@@ -613,12 +613,58 @@ ttc_status_t build_rc_params(const Geometry& g, const int* tile, int ta, int tb,
   p.dC = make_fastdiv((uint32_t)p.C);
   p.dCl = make_fastdiv((uint32_t)p.Cl);
   p.RP = (int32_t)(R | 1);
-  // tile grid: dimensions with more than one tile, in A or B order
+  // Tile grid: dimensions with more than one tile, in A order (1) or B
+  // order (2).  Grouped B order (3: groups of 8, 4: groups of 16): the row
+  // group's outermost tiled dimension is split into an inner position,
+  // walked first, of G tiles and an outer position at its place in B
+  // order, so that the tiles a round takes together extend each other's
+  // runs on both sides (G along A's rows, the rest along B's columns).
   p.splitA = p.splitB = -1;
   p.grank = 0;
+  p.gouter = -1;
+  p.group = 1;
+  p.gcount = 0;
+  const int G = tile_order == 3 ? 8 : (tile_order == 4 ? 16 : 0);
+  int kg = -1;
+  if (G && ntiles > 0) {
+    int tiled = 0;
+    for (int k = 0; k < r; ++k) tiled += gk[k] > 1;
+    for (int m = na - 1; m >= 0 && kg < 0; --m)
+      if (gk[ga[m]] > 1) kg = ga[m];
+    if (tiled + 1 > kMaxDim) kg = -1;  // no position left for the split
+  }
+  long long grid = ntiles > 0 ? 1 : 0;
+  long long gi = 1, outer = 1;
+  if (kg >= 0) {
+    gi = std::min<long long>(G, gk[kg]);
+    outer = (gk[kg] + gi - 1) / gi;
+    const int pos = p.grank++;
+    p.n[pos] = outer > 1 ? (int32_t)(gi * tile[kg]) : g.size[kg];  // never partial when split
+    p.t[pos] = tile[kg];
+    p.gdiv[pos] = make_fastdiv((uint32_t)gi);
+    p.tsa[pos] = (int64_t)tile[kg] * g.sa[kg];
+    p.tsb[pos] = (int64_t)tile[kg] * g.sb[kg];
+    if (outer == 1 && partA && kg == ka) p.splitA = pos;
+    grid *= gi;
+  }
   for (int q = 0; q < r; ++q) {
-    const int k = tile_order == 2 ? g.perm[q] : q;
+    const int k = tile_order >= 2 ? g.perm[q] : q;
     if (gk[k] <= 1) continue;
+    if (k == kg) {
+      if (outer <= 1) continue;
+      const int pos = p.grank++;
+      p.n[pos] = g.size[k];
+      p.t[pos] = tile[k];
+      p.gdiv[pos] = make_fastdiv((uint32_t)outer);
+      p.tsa[pos] = gi * tile[k] * g.sa[k];
+      p.tsb[pos] = gi * tile[k] * g.sb[k];
+      p.gouter = pos;
+      p.group = (uint32_t)gi;
+      p.gcount = (uint32_t)gk[k];
+      if (partA && k == ka) p.splitA = pos;
+      grid *= outer;
+      continue;
+    }
     const int pos = p.grank++;
     p.n[pos] = g.size[k];
     p.t[pos] = tile[k];
@@ -627,8 +673,10 @@ ttc_status_t build_rc_params(const Geometry& g, const int* tile, int ta, int tb,
     p.tsb[pos] = (int64_t)tile[k] * g.sb[k];
     if (partA && k == ka) p.splitA = pos;
     if (partB && k == kb) p.splitB = pos;
+    grid *= gk[k];
   }
-  p.ntiles = (uint32_t)ntiles;
+  if (grid > INT_MAX) return TTC_ERROR_UNSUPPORTED;
+  p.ntiles = (uint32_t)grid;
   p.grid_order = kernel_grid_order(grid_order);
   p.lds_data_elems = (int32_t)(rc_data_bytes(R, C, lds_elem_bytes(ta, tb)) / lds_elem_bytes(ta, tb));
   p.lds_spare = (int32_t)(p.RP * C);
@@ -818,7 +866,7 @@ extern "C" ttc_status_t ttc_plan_create(ttc_plan_t* plan, ttc_dtype_t type_a, tt
   std::memset(&c, 0, sizeof(c));
   if (cfg) c = *cfg;
   if (c.nt_store < 0 || c.nt_store > 2 || c.grid_order < 0 || c.grid_order > 4 || c.wg_per_cu < 0 || c.vec_a < 0 ||
-      c.vec_b < 0 || c.family < 0 || c.family > 3 || c.tile_order < 0 || c.tile_order > 2)
+      c.vec_b < 0 || c.family < 0 || c.family > 3 || c.tile_order < 0 || c.tile_order > 4)
     return TTC_ERROR_INVALID_VALUE;
   ttc_plan_s* pl = new (std::nothrow) ttc_plan_s();
   if (!pl) return TTC_ERROR_ALLOC;

@@ -115,7 +115,8 @@ struct RCWorker {
     uint32_t rem = id;
     c.bA = 0;
     c.bB = 0;
-    bool lastA = false, lastB = false;
+    bool lastA = false, lastB = false, pad = false;
+    uint32_t x0 = 0;
     const int gr = p.grank;
 #pragma unroll 1
     for (int g = 0; g < gr; ++g) {
@@ -128,7 +129,15 @@ struct RCWorker {
       rem = q;
       c.bA += (int64_t)x * kp->tsa[g];
       c.bB += (int64_t)x * kp->tsb[g];
-      const bool partial = (uint32_t)kp->n[g] - x * (uint32_t)kp->t[g] < (uint32_t)kp->t[g];
+      // tile index along the dimension: a split dimension's outer position
+      // adds the inner one (position 0) back; past its tile count: padding
+      uint32_t xa = x;
+      if (g == 0) x0 = x;
+      if (g == p.gouter) {
+        xa = x0 + p.group * x;
+        pad |= xa >= p.gcount;
+      }
+      const bool partial = (uint32_t)kp->n[g] - xa * (uint32_t)kp->t[g] < (uint32_t)kp->t[g];
       lastA |= g == p.splitA && partial;
       lastB |= g == p.splitB && partial;
     }
@@ -138,7 +147,7 @@ struct RCWorker {
     c.cols = lastB ? (uint32_t)p.Cl : (uint32_t)p.C;
     const FastDiv dr = lastA ? p.dRl : p.dR;
     const FastDiv dc = lastB ? p.dCl : p.dC;
-    finish(c, dr, dc, empty);
+    finish(c, dr, dc, empty || pad);
     return c;
   }
 
