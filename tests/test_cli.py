@@ -168,3 +168,37 @@ def test_command_line_entry_point(workdir):
                        cwd=str(workdir))
     assert r.returncode == 0, r.stdout
     assert (workdir / "ttc_transpositions" / "sbCuTranspose_021_8x9x10_bz.hip").exists()
+
+
+def test_database_analytics(tmp_path):
+    """Per-problem best bandwidths, winning configurations and the search's
+    gain over the planner, from a database written by the insert helpers."""
+    from ttc_amd import analytics
+    db = str(tmp_path / "ttc.db")
+    con = sqlite3.connect(db)
+    cur = con.cursor()
+    ttc.createTables(cur)
+    lid = ttc.insertIntoLoopOrder(cur, [1, 0])
+    cases = [([1, 0], [64, 48], "float", [(None, 1, 2, 1000.0), ({"tile": [32, 32]}, 1, 1, 1500.0)]),
+             ([1, 0], [96, 80], "float", [(None, 2, 2, 2000.0), ({"tile": [16, 16]}, 1, 1, 1000.0)]),
+             ([2, 0, 1], [8, 9, 10], "double", [(None, 2, 2, 800.0), ({"tile": [8, 9, 2]}, 2, 1, 1200.0)])]
+    for perm, size, t, variants in cases:
+        sid = ttc.insertIntoSize(cur, size)
+        pid = ttc.insertIntoPermutation(cur, perm)
+        mid = ttc.insertIntoMeasurements(cur, len(size), "h", 0, 1.0, 0.0, 0, "hipcc", t, t, 0, 0, 0, 0, sid, pid,
+                                         -1, 0, 0, "gfx950")
+        for rank, (cfg, fam, tord, bw) in enumerate(variants):
+            info = {"run_a": 1, "run_b": 1, "tile": size, "vec_a": 4, "vec_b": 4, "threads": 256, "nt_store": 0,
+                    "grid_order": 4, "family": fam, "tile_order": tord}
+            ttc.insertIntoVariant(cur, mid, lid, rank, {"config": cfg or {}, "ms": 0.1, "GiBs": bw, "info": info})
+    con.commit()
+    con.close()
+    s = analytics.summarize(db)
+    assert s["measurements"] == 3
+    rows = {(r["pair"], r["rank"]): r for r in s["by_problem"]}
+    assert rows[("float->float", 2)]["n"] == 2 and rows[("float->float", 2)]["median"] == 1750.0
+    assert rows[("double->double", 3)]["max"] == 1200.0
+    assert s["planner"] == {"measurements": 3, "planner_fastest": 1, "median_gain": 1.5, "max_gain": 1.5}
+    assert s["winners"][0]["count"] >= 1 and sum(w["count"] for w in s["winners"]) == 3
+    text = analytics.render(s)
+    assert "row x column" in text and "1 of 3 searches" in text

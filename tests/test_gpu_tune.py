@@ -69,3 +69,27 @@ def test_autotune_fp8_candidates_agree(gpu):
         torch.cuda.synchronize()
         assert same_bits(dB.cpu().numpy(), want), r["config"]
         assert float(amax.item()) == want_amax
+
+
+def test_hot_cache_modes_time_and_key_the_database(gpu, tmp_path, monkeypatch):
+    """--hotA/--hotB: the search times with A (B) reused between launches and
+    stores the result under its own database key."""
+    from ttc_amd import autotune, sql_util, ttc, ttc_util
+    import sqlite3
+    for hot_a, hot_b in ((True, False), (False, True), (True, True)):
+        res = autotune.tune(S, S, [1, 0], [300, 200], alpha=1.0, beta=1.0, max_impl=3, reps=2, hot_a=hot_a,
+                            hot_b=hot_b)
+        assert res and all(r["GBs"] > 0 for r in res)
+    monkeypatch.setenv("TTC_ROOT", str(tmp_path))
+    monkeypatch.chdir(tmp_path)
+    for hot in (0, 1):
+        a = ttc_util.TTCargs([1, 0], [300, 200])
+        a.floatTypeA = a.floatTypeB = "float"
+        a.alpha, a.beta = 1.0, 1.0
+        a.maxNumImplementations = 3
+        a.silent = 1
+        a.hotA = hot
+        ttc.generateTransposition(a)
+    con = sqlite3.connect(os.path.join(str(tmp_path), "ttc", "ttc.db"))
+    assert sorted(r[0] for r in con.execute("SELECT hotA FROM measurements")) == [0, 1]
+    con.close()

@@ -162,10 +162,13 @@ def _device_inputs(torch, type_a, type_b, perm, size, lda, ldb, dev):
 
 
 def tune(type_a, type_b, perm, size, lda=None, ldb=None, alpha=1.0, beta=0.0, max_impl=200, reps=5,
-         progress=None, nt_stores=False):
+         progress=None, nt_stores=False, hot_a=False, hot_b=False):
     """Times every candidate on device buffers of the problem's shape.
     Returns results sorted fastest first: dicts with config, info, ms, GBs
-    (10^9 B/s) and GiBs (2^30 B/s, the reference's unit)."""
+    (10^9 B/s) and GiBs (2^30 B/s, the reference's unit).  hot_a / hot_b:
+    the reference's cache modes (ttc/transposeGenerator.py:648-651,
+    744-747): every launch of a round reuses the same A (resp. B), which
+    therefore stays in cache when it fits, instead of a fresh buffer."""
     import torch
     dev = torch.device("cuda:0")
     bz = beta == 0
@@ -175,7 +178,11 @@ def tune(type_a, type_b, perm, size, lda=None, ldb=None, alpha=1.0, beta=0.0, ma
     # last-level cache, and at least four launches per round
     footprint = A.numel() + B.numel()
     while (footprint * len(pairs) < (1536 << 20) or len(pairs) < 4) and len(pairs) < 16:
-        pairs.append(_device_inputs(torch, type_a, type_b, perm, size, lda, ldb, dev))
+        if hot_a and hot_b:
+            pairs.append((A, B))
+        else:
+            A2, B2 = _device_inputs(torch, type_a, type_b, perm, size, lda, ldb, dev)
+            pairs.append((A if hot_a else A2, B if hot_b else B2))
     results = []
     cands = candidates(type_a, type_b, perm, size, lda, ldb, bz, max_impl, nt_stores)
     for i, cfg in enumerate(cands):

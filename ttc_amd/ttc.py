@@ -81,7 +81,7 @@ def printHelp():
     print("   --generateOnly".ljust(60), "emit the heuristic choice, no timing")
     print("   --keep".ljust(60), "accepted for compatibility")
     print("   --ignoreDatabase".ljust(60), "don't use the SQL database (i.e., no lookup)")
-    print("   --hotA / --hotB".ljust(60), "database key fields (kept for compatibility)")
+    print("   --hotA / --hotB".ljust(60), "time with A (B) kept in cache between launches (database keys)")
     print("   --verbose or -v".ljust(60), "verbose output")
     print("   --help".ljust(60), "prints this help")
     print("")
@@ -102,7 +102,8 @@ def database_path():
 
 EXTRA_VARIANT_COLUMNS = [("tile", "VARCHAR(64)"), ("vecA", "INTEGER"), ("vecB", "INTEGER"),
                          ("threadsPerBlock", "INTEGER"), ("wgPerCU", "INTEGER"), ("ntStore", "INTEGER"),
-                         ("gridOrder", "INTEGER"), ("runA", "INTEGER"), ("runB", "INTEGER"), ("config", "TEXT")]
+                         ("gridOrder", "INTEGER"), ("runA", "INTEGER"), ("runB", "INTEGER"), ("config", "TEXT"),
+                         ("family", "INTEGER"), ("tileOrder", "INTEGER")]
 
 
 def createTables(cursor):
@@ -213,11 +214,12 @@ def insertIntoVariant(cursor, measurement_id, loop_id, rank, result):
     info, cfg = result["info"], result["config"]
     cursor.execute("""INSERT INTO variant (blockA, blockB, prefetchDistance, bandwidth, time, tlbMisses, l2misses,
         rankLoopOrder, rankBlocking, measurement_id, loopOrder_id, tile, vecA, vecB, threadsPerBlock, wgPerCU,
-        ntStore, gridOrder, runA, runB, config) VALUES (?,?,?,?,?,?,?,?,?,?,?,?,?,?,?,?,?,?,?,?,?)""",
+        ntStore, gridOrder, runA, runB, config, family, tileOrder)
+        VALUES (?,?,?,?,?,?,?,?,?,?,?,?,?,?,?,?,?,?,?,?,?,?,?)""",
                    (info["run_a"], info["run_b"], 0, result["GiBs"], result["ms"] * 1e-3, -1.0, -1.0, rank, rank,
                     measurement_id, loop_id, "x".join(str(t) for t in info["tile"]), info["vec_a"], info["vec_b"],
                     info["threads"], cfg.get("wg_per_cu", 0), info["nt_store"], info["grid_order"], info["run_a"],
-                    info["run_b"], json.dumps(cfg)))
+                    info["run_b"], json.dumps(cfg), info.get("family", 0), info.get("tile_order", 0)))
     return cursor.lastrowid
 
 
@@ -333,7 +335,7 @@ def generateTransposition(ttcArgs):
                 sys.stdout.flush()
         cands_limit = ttcArgs.maxNumImplementations
         results = autotune.tune(ta, tb, ttcArgs.idxPerm, ttcArgs.size, lda, ldb, ttcArgs.alpha, ttcArgs.beta,
-                                cands_limit, progress=progress)
+                                cands_limit, progress=progress, hot_a=bool(ttcArgs.hotA), hot_b=bool(ttcArgs.hotB))
         measuringTime = _time.time() - t0
         if not results:
             _fail("no variant could be measured")
