@@ -83,6 +83,8 @@ typedef struct {
                             3 = row copy (perm[0] == 0 after fusion only) */
   int tile_order;        /* tile numbering (row x column tiles): 0 = auto (B order), 1 = A order, 2 = B order,
                             3 / 4 = B order in groups of 8 / 16 tiles along A's leading tiled dimension */
+  int lds_buffers;       /* LDS tile buffers (row x column tiles): 0 = auto (2), 1 = one (two barriers per
+                            tile, more resident work-groups), 2 = two (one barrier per tile) */
 } ttc_config_t;
 
 /* What a plan resolved to (fused geometry, tile, launch shape). */
@@ -110,7 +112,8 @@ typedef struct {
   int run_a, run_b;              /* contiguous element runs of a full tile in A and B */
   int family;                    /* 1 = box tile kernel, 2 = row x column tile kernel, 3 = row copy */
   int rows, cols;                /* row x column tile: row and column counts (0 for box tiles) */
-  int tile_order;                /* 1 = A order, 2 = B order */
+  int tile_order;                /* 1 = A order, 2 = B order, 3 / 4 grouped B order */
+  int lds_buffers;               /* LDS tile buffers of the row x column kernel (1 for other families) */
 } ttc_plan_info_t;
 
 /* Library version string. */

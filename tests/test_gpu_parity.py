@@ -97,7 +97,7 @@ def test_random_shapes(gpu, code):
             for beta in (0.0, 0.7):
                 order = 1 + (i + 2 * int(beta > 0)) % 4  # row x column tiles: A, B and grouped B numbering
                 info = check(gpu, tA, tB, perm, size, 1.3, beta, seed=100 + i, dist="outliers",
-                             config=dict(family=fam, tile_order=order))
+                             config=dict(family=fam, tile_order=order, lds_buffers=1 + (i + int(beta > 0)) % 2))
                 assert info["family"] == fam and info["tile_order"] == (order if fam == 2 else 1)
     assert n_rc >= 9 and n_row >= 4
 
@@ -158,7 +158,7 @@ def test_explicit_row_column_tiles(gpu):
     for perm, size, tile in cases:
         for code in ("s", "d", "z", "bs"):
             tA, tB = CODE[code]
-            cfg = dict(tile=tile, family=2, vec_a=1, vec_b=1, tile_order=1 + n % 4)
+            cfg = dict(tile=tile, family=2, vec_a=1, vec_b=1, tile_order=1 + n % 4, lds_buffers=1 + n % 2)
             # volume limit of the family: 256 threads x (128 bytes, at most 32 elements) each side
             limit = 256 * min(min(32, 128 // rt.DTYPE_BYTES[t]) for t in (tA, tB))
             if int(np.prod(tile)) > limit:

@@ -131,7 +131,8 @@ def _check_plan(i, perm, size):
         assert i["slots_a"] * i["vec_a"] == ea and i["slots_b"] * i["vec_b"] == eb
         assert V <= 256 * min(ea, eb)
         _check_rc_groups(i)
-    assert 0 < i["lds_bytes"] <= 65536
+    assert 0 < i["lds_bytes"] <= (81920 if i["family"] == 2 else 65536)
+    assert i["lds_buffers"] == (2 if i["family"] == 2 else 1)
     partial = sum(1 for k in range(r) if i["size"][k] % i["tile"][k])
     assert partial <= 2
     assert i["elements"] == int(np.prod(size))
@@ -234,6 +235,18 @@ def test_autotune_candidates_cover_families_orders_and_waves():
     for c in cands:
         rt.Plan(S, S, perm, size, beta_is_zero=False, config=c or None).info()
     assert len(autotune.candidates(S, S, perm, size, max_impl=1)) == 1
+
+
+def test_lds_buffers():
+    """Row x column plans stage through two LDS tile buffers by default and
+    one on request; the second buffer costs exactly one more data region."""
+    one = rt.Plan(S, S, [1, 0], [3000, 2000], config={"family": 2, "lds_buffers": 1}).info()
+    two = rt.Plan(S, S, [1, 0], [3000, 2000], config={"family": 2}).info()
+    assert one["lds_buffers"] == 1 and two["lds_buffers"] == 2 and one["tile"] == two["tile"]
+    tables = (2 * one["rows"] + 2 * one["cols"]) * 4
+    assert two["lds_bytes"] - tables == 2 * (one["lds_bytes"] - tables)
+    with pytest.raises(rt.TTCError):
+        rt.Plan(S, S, [1, 0], [3000, 2000], config={"lds_buffers": 3})
 
 
 def test_batch_op_bytes():

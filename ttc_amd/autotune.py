@@ -115,6 +115,12 @@ def candidates(type_a, type_b, perm, size, lda=None, ldb=None, beta_is_zero=Fals
                 c.update(launch[0])
                 c["wg_per_cu"] = w
                 out.append(c)
+        # one LDS buffer (two barriers per tile, more resident work-groups)
+        if len(out) < max_impl and ranked[0]["family"] == 2:
+            c = dict(ranked[0])
+            c.update(launch[0])
+            c["lds_buffers"] = 1
+            out.append(c)
     return out[:max(1, max_impl)]
 
 

@@ -172,6 +172,7 @@ __device__ __forceinline__ void scalar_rc(KRCOp& o, RCParams& p) {
   p.colB_dense = o.p.colB_dense;
   p.lds_data_elems = o.p.lds_data_elems;
   p.lds_spare = o.p.lds_spare;
+  p.nbuf = o.p.nbuf;  // places the offset tables in LDS (setup_tables)
 }
 
 template <class OP, int VA, int VB, bool BETA>
@@ -433,6 +434,7 @@ extern "C" ttc_status_t ttc_batch_create(ttc_batch_t* out, int nops, const ttc_b
     // vector widths: 1 or the widest of the type (the kinds instantiated)
     ttc_config_t cfg = op.cfg;
     if (cfg.tile_order > 2) cfg.tile_order = 2;  // grouped launches walk host-built tile records in B order
+    cfg.lds_buffers = 1;                          // and stage every tile through one LDS buffer
     const int wide = std::max(1, 16 / dtype_bytes(op.type_a));
     for (int pass = 0; pass < 2; ++pass) {
       ttc_plan_t pl = nullptr;

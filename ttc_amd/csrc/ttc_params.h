@@ -104,6 +104,7 @@ struct RCParams {
   // is its number of real tiles, ids past it are empty padding.
   int32_t gouter;
   uint32_t group, gcount;
+  int32_t nbuf;            // LDS tile buffers: 2 lets a tile's staging overlap the previous tile's reads
   uint32_t ntiles;         // grid size (with the padding of a grouped walk)
   int32_t grid_order;
   int32_t lds_data_elems;  // tile data region in LDS elements (RP * C, a spare vector slot, 16-byte rounding)

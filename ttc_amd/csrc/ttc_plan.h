@@ -92,7 +92,7 @@ int max_tile_volume(int ta, int tb, int va, int vb);
 int max_rc_volume(int ta, int tb);
 bool rc_groups(const Geometry& g, const int* tile, int& na, int* ga, int& nb, int* gb);
 ttc_status_t build_rc_params(const Geometry& g, const int* tile, int ta, int tb, int va, int vb, int grid_order,
-                             int tile_order, RCParams& p, int& lds_bytes);
+                             int tile_order, RCParams& p, int& lds_bytes, int nbuf = 1);
 bool choose_rc_tile(const Geometry& g, int ta, int tb, int va, int vb, int run_a, int run_b, int* tile,
                     double* score);
 
@@ -114,7 +114,8 @@ struct ttc_plan_s {
   int lds_bytes, lds_pad;
   int run_a, run_b;
   int family;            // 1 box tile kernel (p), 2 row x column kernel (rc), 3 row copy (row)
-  int tile_order;        // 1 A order, 2 B order (row x column tiles only)
+  int tile_order;        // 1 A order, 2 B order, 3 / 4 grouped B order (row x column tiles only)
+  int lds_buffers;       // LDS tile buffers of the row x column kernel
   ttc::RCParams rc;
   ttc::RowParams row;
   ttc_plan_s* fallback;  // scalar-vector plan for misaligned pointers (may be null)

@@ -69,7 +69,7 @@ struct RCWorker {
   __device__ __forceinline__ void setup(const RCParams& p, unsigned char* lds_raw, uint32_t tid_) {
     tid = tid_;
     data = reinterpret_cast<L*>(lds_raw);
-    int32_t* tab = reinterpret_cast<int32_t*>(lds_raw + (size_t)p.lds_data_elems * sizeof(L));
+    int32_t* tab = reinterpret_cast<int32_t*>(lds_raw + (size_t)p.nbuf * (size_t)p.lds_data_elems * sizeof(L));
     int32_t* rA = tab;
     int32_t* rB = tab + p.R;
     int32_t* cA = tab + 2 * p.R;
@@ -158,7 +158,7 @@ struct RCWorker {
                                                unsigned char* lds_raw, uint32_t tid_) {
     tid = tid_;
     data = reinterpret_cast<L*>(lds_raw);
-    int32_t* t = reinterpret_cast<int32_t*>(lds_raw + (size_t)p.lds_data_elems * sizeof(L));
+    int32_t* t = reinterpret_cast<int32_t*>(lds_raw + (size_t)p.nbuf * (size_t)p.lds_data_elems * sizeof(L));
     const int n = 2 * p.R + 2 * p.C;
     for (int i = (int)tid; i < n; i += kRcThreads) t[i] = tab[i];
     rowA = t;
@@ -323,7 +323,15 @@ struct RCWorker {
     TB ob[NIB][VB];
     Tile c = tile(p, kp, first, false);
     load(p, c, A, va);
+    // Two LDS buffers (nbuf == 2): tile k stages into buffer k mod 2, so the
+    // barrier after staging also orders the previous use of that buffer
+    // (the reads of tile k-2, finished before barrier k-1) and the second
+    // barrier goes.  One buffer: the next staging waits for this tile's reads.
+    L* const base = data;
+    const uint32_t alt = p.nbuf == 2 ? (uint32_t)p.lds_data_elems : 0u;
+    uint32_t off = 0;
     for (uint32_t t = first; t < end; t += step) {
+      data = base + off;
       stage(p, c, va, alpha);
       __syncthreads();
       if constexpr (BETA) load_b(p, c, B, ob);
@@ -331,9 +339,11 @@ struct RCWorker {
       const Tile cn = tile(p, kp, tn < end ? tn : t, tn >= end);
       load(p, cn, A, va);
       store(p, c, B, beta, ob);
-      __syncthreads();
+      if (alt == 0) __syncthreads();
+      off = alt - off;
       c = cn;
     }
+    data = base;
   }
 };
 

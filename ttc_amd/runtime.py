@@ -37,24 +37,24 @@ class Config(ctypes.Structure):
     _fields_ = [("tile", ctypes.c_int * MAX_DIM), ("vec_a", ctypes.c_int), ("vec_b", ctypes.c_int),
                 ("wg_per_cu", ctypes.c_int), ("nt_store", ctypes.c_int), ("grid_order", ctypes.c_int),
                 ("run_a", ctypes.c_int), ("run_b", ctypes.c_int), ("family", ctypes.c_int),
-                ("tile_order", ctypes.c_int)]
+                ("tile_order", ctypes.c_int), ("lds_buffers", ctypes.c_int)]
 
     @classmethod
     def make(cls, tile=None, vec_a=0, vec_b=0, wg_per_cu=0, nt_store=0, grid_order=0, run_a=0, run_b=0, family=0,
-             tile_order=0):
+             tile_order=0, lds_buffers=0):
         c = cls()
         if tile:
             for k, t in enumerate(tile):
                 c.tile[k] = int(t)
         c.vec_a, c.vec_b, c.wg_per_cu, c.nt_store = int(vec_a), int(vec_b), int(wg_per_cu), int(nt_store)
         c.grid_order, c.run_a, c.run_b, c.family = int(grid_order), int(run_a), int(run_b), int(family)
-        c.tile_order = int(tile_order)
+        c.tile_order, c.lds_buffers = int(tile_order), int(lds_buffers)
         return c
 
     def as_dict(self):
         return {"tile": list(self.tile), "vec_a": self.vec_a, "vec_b": self.vec_b, "wg_per_cu": self.wg_per_cu,
                 "nt_store": self.nt_store, "grid_order": self.grid_order, "run_a": self.run_a, "run_b": self.run_b,
-                "family": self.family, "tile_order": self.tile_order}
+                "family": self.family, "tile_order": self.tile_order, "lds_buffers": self.lds_buffers}
 
 
 class PlanInfo(ctypes.Structure):
@@ -67,7 +67,8 @@ class PlanInfo(ctypes.Structure):
                 ("wg_per_cu", ctypes.c_int), ("elements", ctypes.c_int64), ("bytes_algorithmic", ctypes.c_int64),
                 ("type_a", ctypes.c_int), ("type_b", ctypes.c_int), ("beta_is_zero", ctypes.c_int),
                 ("run_a", ctypes.c_int), ("run_b", ctypes.c_int), ("family", ctypes.c_int),
-                ("rows", ctypes.c_int), ("cols", ctypes.c_int), ("tile_order", ctypes.c_int)]
+                ("rows", ctypes.c_int), ("cols", ctypes.c_int), ("tile_order", ctypes.c_int),
+                ("lds_buffers", ctypes.c_int)]
 
     def as_dict(self):
         r = self.rank
@@ -80,7 +81,7 @@ class PlanInfo(ctypes.Structure):
                 "elements": self.elements, "bytes_algorithmic": self.bytes_algorithmic,
                 "type_a": self.type_a, "type_b": self.type_b, "beta_is_zero": self.beta_is_zero,
                 "run_a": self.run_a, "run_b": self.run_b, "family": self.family, "rows": self.rows,
-                "cols": self.cols, "tile_order": self.tile_order}
+                "cols": self.cols, "tile_order": self.tile_order, "lds_buffers": self.lds_buffers}
 
 
 class BatchOp(ctypes.Structure):
