@@ -116,9 +116,11 @@ def test_padded_subtensor(gpu):
             tA, tB = CODE[code]
             for fam in families(tA, tB, perm, size, lda, ldb):
                 cfg = dict(family=fam)
-                check(gpu, tA, tB, perm, size, 1.0, 1.0, lda, ldb, seed=i, config=cfg)
+                # row x column tiles: grouped B order and one LDS buffer on the padded cases too
+                extra = dict(tile_order=3 + i % 2, lds_buffers=1 + i % 2) if fam == 2 else {}
+                check(gpu, tA, tB, perm, size, 1.0, 1.0, lda, ldb, seed=i, config=dict(cfg, **extra))
                 check(gpu, tA, tB, perm, size, 1.1, 0.0, lda, None, seed=i, config=cfg)
-                check(gpu, tA, tB, perm, size, 0.9, -1.0, None, ldb, seed=i, config=cfg)
+                check(gpu, tA, tB, perm, size, 0.9, -1.0, None, ldb, seed=i, config=dict(cfg, **extra))
 
 
 def test_every_vector_width_and_mode(gpu):
