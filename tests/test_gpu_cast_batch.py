@@ -169,6 +169,15 @@ def test_batch_grouped_and_loop_match_oracle(gpu):
         got = from_device(o["B"], o["hB"])
         assert same_bits(got, o["want"]), (i, o["type_a"], o["perm"], o["size"], o["alpha"], o["beta"],
                                            first_diff(got, o["want"]))
+    # row x column ops staged through two LDS buffers inside the grouped launch
+    for o in ops:
+        o["B"].copy_(to_device(o["hB"], gpu))
+    torch.cuda.synchronize()
+    batch2 = rt.Batch([dict(o, config={"lds_buffers": 2}) for o in ops])
+    batch2.execute()
+    torch.cuda.synchronize()
+    for i, o in enumerate(ops):
+        assert same_bits(from_device(o["B"], o["hB"]), o["want"]), ("two LDS buffers", i)
     # the per-op launch loop with the same kernels gives the same bits
     for o in ops:
         o["B"].copy_(to_device(o["hB"], gpu))

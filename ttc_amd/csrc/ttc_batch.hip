@@ -434,7 +434,6 @@ extern "C" ttc_status_t ttc_batch_create(ttc_batch_t* out, int nops, const ttc_b
     // vector widths: 1 or the widest of the type (the kinds instantiated)
     ttc_config_t cfg = op.cfg;
     if (cfg.tile_order > 2) cfg.tile_order = 2;  // grouped launches walk host-built tile records in B order
-    cfg.lds_buffers = 1;                          // and stage every tile through one LDS buffer
     const int wide = std::max(1, 16 / dtype_bytes(op.type_a));
     for (int pass = 0; pass < 2; ++pass) {
       ttc_plan_t pl = nullptr;

@@ -363,7 +363,13 @@ __device__ __forceinline__ void rc_run_table(RCWorker<OP, VA, VB, BETA, NT>& w, 
   typename OP::TB ob[W::NIB][VB];
   typename W::Tile c = w.tile_rec(p, recs[lo], false);
   w.load(p, c, A, va);
+  // LDS buffers as in RCWorker::run; with two, one barrier at the end orders
+  // this op's last reads before the next op rewrites the offset tables.
+  typename OP::L* const base = w.data;
+  const uint32_t alt = p.nbuf == 2 ? (uint32_t)p.lds_data_elems : 0u;
+  uint32_t off = 0;
   for (uint32_t t = lo; t < hi; ++t) {
+    w.data = base + off;
     w.stage(p, c, va, alpha);
     __syncthreads();
     if constexpr (BETA) w.load_b(p, c, B, ob);
@@ -371,9 +377,12 @@ __device__ __forceinline__ void rc_run_table(RCWorker<OP, VA, VB, BETA, NT>& w, 
     const typename W::Tile cn = w.tile_rec(p, recs[tn], t + 1 >= hi);
     w.load(p, cn, A, va);
     w.store(p, c, B, beta, ob);
-    __syncthreads();
+    if (alt == 0) __syncthreads();
+    off = alt - off;
     c = cn;
   }
+  w.data = base;
+  if (alt != 0) __syncthreads();
 }
 
 // The parameters are the kernel's first argument, so they sit at the start
