@@ -434,6 +434,7 @@ extern "C" ttc_status_t ttc_batch_create(ttc_batch_t* out, int nops, const ttc_b
     // vector widths: 1 or the widest of the type (the kinds instantiated)
     ttc_config_t cfg = op.cfg;
     if (cfg.tile_order > 2) cfg.tile_order = 2;  // grouped launches walk host-built tile records in B order
+    if (cfg.lds_buffers == 0) cfg.lds_buffers = 1;  // one LDS buffer unless the op asks for two (measured faster)
     const int wide = std::max(1, 16 / dtype_bytes(op.type_a));
     for (int pass = 0; pass < 2; ++pass) {
       ttc_plan_t pl = nullptr;
