@@ -468,7 +468,7 @@ def main():
     step_bytes = sum(o["bytes"] for o in ops)
     value = step_bytes * args.steps / (total_ms * 1e-3) / 1e9
     # per-op and per-kernel times (diagnostic pass, each launch between events)
-    per_op = per_op_ms(torch, ops, stream, 3)
+    per_op = per_op_ms(torch, ops, stream, 5)
     groups = {}
     for r in per_op:
         for i, o in enumerate(ops):
