@@ -82,8 +82,15 @@ def test_generate_only_emits_compilable_drop_in(workdir):
     assert name == "sCuTranspose_102_1000x768x16" and bw == 0
     src = (workdir / "ttc_transpositions" / (name + ".hip")).read_text()
     hdr = (workdir / "ttc_transpositions" / (name + ".h")).read_text()
-    assert 'extern "C" void sCuTranspose_102_1000x768x16(const float *A, float *B, const float alpha, ' \
-           'const float beta, const int *size, const int *lda, const int *ldb)' in hdr
+    assert '\nvoid sCuTranspose_102_1000x768x16(const float *A, float *B, const float alpha, ' \
+           'const float beta, const int *size, const int *lda, const int *ldb);' in hdr
+    assert '#ifdef __cplusplus\nextern "C" {\n#endif' in hdr  # C and C++ callers
+    assert 'extern "C" void sCuTranspose_102_1000x768x16(const float *A' in src
+    # the header compiles as C (TTC-C's language) as well as C++
+    r = subprocess.run(["gcc", "-x", "c", "-fsyntax-only", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
+                        str(workdir / "ttc_transpositions" / (name + ".h"))], stdout=subprocess.PIPE,
+                       stderr=subprocess.STDOUT, text=True)
+    assert r.returncode == 0, r.stdout
     assert "ttc_legacy_transpose(TTC_S, TTC_S" in src
     obj = workdir / "a.o"
     r = subprocess.run(["hipcc", "--offload-arch=gfx950", "-c", str(workdir / "ttc_transpositions" / (name + ".hip")),
@@ -202,3 +209,62 @@ def test_database_analytics(tmp_path):
     assert s["winners"][0]["count"] >= 1 and sum(w["count"] for w in s["winners"]) == 3
     text = analytics.render(s)
     assert "row x column" in text and "1 of 3 searches" in text
+
+
+def test_stored_solution_for_other_leading_dims_is_not_reused(workdir):
+    """The database key holds no lda/ldb (as the reference's); a
+    configuration tuned on the dense problem whose vector width a padded
+    lda rules out is dropped and the planner's choice emitted instead."""
+    from ttc_amd import runtime as rt
+    dense = rt.Plan(rt.S, rt.S, [1, 0], [1000, 768], beta_is_zero=False)
+    cfg = {k: dense.info()[k] for k in ("tile", "vec_a", "vec_b", "family")}
+    assert cfg["vec_a"] > 1  # a wide vector along A's 1000 rows
+    with pytest.raises(rt.TTCError):
+        rt.Plan(rt.S, rt.S, [1, 0], [1000, 768], [1001, 768], None, False, cfg)
+    db = ttc.database_path()
+    os.makedirs(os.path.dirname(db), exist_ok=True)
+    con = sqlite3.connect(db)
+    cur = con.cursor()
+    ttc.createTables(cur)
+    sid = ttc.insertIntoSize(cur, [1000, 768])
+    pid = ttc.insertIntoPermutation(cur, [1, 0])
+    mid = ttc.insertIntoMeasurements(cur, 2, "h", 0, 1.0, 1.0, 0, "hipcc", "float", "float", 0, 0, 0, 0, sid, pid,
+                                     -1, 0, 0, ttc_util.getArchitecture("hip"))
+    lid = ttc.insertIntoLoopOrder(cur, [1, 0])
+    info = dict(dense.info(), threads=256)
+    ttc.insertIntoVariant(cur, mid, lid, 0, {"config": cfg, "ms": 0.1, "GiBs": 999.0, "info": info})
+    con.commit()
+    con.close()
+    # the dense call is served from the database ...
+    name, bw = ttc.generateTransposition(ttc.parse_args(["--perm=1,0", "--size=1000,768", "--beta=1"]))
+    assert bw == 999.0
+    # ... the padded one falls back to the planner (no GPU: --generateOnly)
+    a = ttc.parse_args(["--perm=1,0", "--size=1000,768", "--beta=1", "--lda=1001,768", "--generateOnly"])
+    name, bw = ttc.generateTransposition(a)
+    assert bw == 0.0
+    src = (workdir / "ttc_transpositions" / (name + ".hip")).read_text()
+    padded = rt.Plan(rt.S, rt.S, [1, 0], [1000, 768], [1001, 768], None, False).info()
+    assert "{{%s}, %d, %d" % (", ".join(str(t) for t in padded["tile"] + [0] * 6), padded["vec_a"],
+                               padded["vec_b"]) in src
+
+
+def test_batch_rejects_short_buffers_before_any_device_call():
+    """Batch checks every op's buffers against the extent its perm, size and
+    lda/ldb span (the C ABI takes raw pointers and cannot), and keeps the
+    tensors alive for replays."""
+    import torch
+    from oracle import oracle as orc
+    from ttc_amd import runtime as rt
+    perm, size, lda, ldb = [2, 0, 1], [5, 6, 7], [6, 6, 7], [7, 6, 8]
+    sA, sB = orc.strides(perm, size, lda, ldb)
+    last_a = sum((n - 1) * s for n, s in zip(size, sA))  # offset of the last element
+    last_b = sum((n - 1) * s for n, s in zip(size, sB))
+    assert rt.extent_bytes(rt.S, rt.D, perm, size, lda, ldb) == ((last_a + 1) * 4, (last_b + 1) * 8)
+    na, nb = last_a + 1, last_b + 1
+    assert rt.extent_bytes(rt.S, rt.S, [1, 0], [0, 9]) == (0, 0)
+    good = dict(type_a=rt.S, type_b=rt.S, perm=perm, size=size, lda=lda, ldb=ldb,
+                A=torch.zeros(na), B=torch.zeros(nb))
+    with pytest.raises(ValueError, match="buffer B"):
+        rt.Batch([good, dict(good, B=torch.zeros(nb - 1))])
+    with pytest.raises(ValueError, match="buffer A"):
+        rt.Batch([dict(good, A=torch.zeros(2 * na)[::2])])  # not contiguous

@@ -750,7 +750,7 @@ bool choose_rc_tile(const Geometry& g, int ta, int tb, int va, int vb, int run_a
 // ================================================================== C ABI
 using namespace ttc;
 
-extern "C" const char* ttc_version(void) { return "ttc-mi355x 0.1.0 (gfx950)"; }
+// ttc_version() is generated with a digest of the sources (gen/ttc_version.cpp).
 
 extern "C" const char* ttc_status_string(ttc_status_t s) {
   switch (s) {

@@ -272,6 +272,20 @@ extern "C" void ttc_legacy_transpose(ttc_dtype_t type_a, ttc_dtype_t type_b, con
       plan = it->second;
     } else {
       ttc_status_t st = ttc_plan_create(&plan, type_a, type_b, dim, perm, size, lda, ldb, beta_is_zero, cfg);
+      if (st != TTC_SUCCESS && cfg) {
+        // A configuration baked into an emitted artefact was chosen for the
+        // lda/ldb it was tuned on; the wrapper takes lda/ldb at call time and
+        // the reference kernels accept any.  Keep the launch settings and
+        // let the planner re-derive the tile and vector widths, then drop
+        // the configuration altogether.
+        ttc_config_t relaxed = *cfg;
+        for (int k = 0; k < TTC_MAX_DIM; ++k) relaxed.tile[k] = 0;
+        relaxed.vec_a = relaxed.vec_b = 0;
+        relaxed.run_a = relaxed.run_b = 0;
+        st = ttc_plan_create(&plan, type_a, type_b, dim, perm, size, lda, ldb, beta_is_zero, &relaxed);
+        if (st != TTC_SUCCESS)
+          st = ttc_plan_create(&plan, type_a, type_b, dim, perm, size, lda, ldb, beta_is_zero, nullptr);
+      }
       if (st != TTC_SUCCESS) {
         printf("\nKernel ERROR in %s: %s (line: %d)\n", who, ttc_status_string(st), __LINE__);
         exit(-1);

@@ -80,11 +80,15 @@ def render(floatTypeA, floatTypeB, perm, size, beta, cfg, note=""):
     fname = function_name(floatTypeA, floatTypeB, perm, size, beta)
     proto = prototype(fname, floatTypeA, floatTypeB, beta)
     guard = (fname + "_H").upper()
+    # The prototype sits in an extern "C" block so that C callers (TTC-C)
+    # can include the header as well as C++ ones.
     hdr = ["#ifndef %s" % guard, "#define %s" % guard, "#include <hip/hip_complex.h>", "#include <stdint.h>",
            "/* B(%s) <- alpha * A(%s)%s */" % (",".join("i%d" % p for p in perm),
                                               ",".join("i%d" % i for i in range(len(perm))),
                                               " + beta * B(...)" if beta != 0 else ""),
-           proto + ";", "#endif", ""]
+           "#ifdef __cplusplus", 'extern "C" {', "#endif",
+           proto[len('extern "C" '):] + ";",
+           "#ifdef __cplusplus", "}", "#endif", "#endif", ""]
     src = ["// Generated by ttc_amd %s%s" % (".".join(str(v) for v in ttc_util.getVersion()),
                                             (" -- " + note) if note else ""),
            "// B[pi(i)] = alpha * A[i]%s with pi = (%s), size = (%s)." % (

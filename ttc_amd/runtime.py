@@ -289,13 +289,54 @@ def legacy_transpose(type_a, type_b, A, B, alpha, beta, perm, size, lda=None, ld
                                  name.encode() if name else None)
 
 
+def extent_bytes(type_a, type_b, perm, size, lda=None, ldb=None):
+    """Bytes spanned by A and by B (last element offset + 1) of an unfused
+    problem: A's stride of dimension k is prod_{m<k} lda[m], B's stride of
+    position q is prod_{m<q} ldb[m] (NULL = dense), as the C ABI reads them."""
+    d = len(size)
+    if any(int(s) == 0 for s in size):
+        return 0, 0
+    LA = list(lda) if lda else list(size)
+    LB = list(ldb) if ldb else [size[perm[q]] for q in range(d)]
+    ea = eb = 1
+    sa = sb = 1
+    for k in range(d):
+        ea += (int(size[k]) - 1) * sa
+        sa *= int(LA[k])
+        eb += (int(size[perm[k]]) - 1) * sb
+        sb *= int(LB[k])
+    return ea * DTYPE_BYTES[int(type_a)], eb * DTYPE_BYTES[int(type_b)]
+
+
+def _check_tensor(name, t, need, where):
+    """A torch tensor handed over as a raw pointer must be contiguous and
+    hold `need` bytes (an int address cannot be checked)."""
+    if t is None or isinstance(t, int):
+        return
+    have = t.numel() * t.element_size()
+    if not t.is_contiguous() or have < need:
+        raise ValueError("%s: buffer %s holds %d bytes (contiguous: %s) but the op spans %d bytes"
+                         % (where, name, have, t.is_contiguous(), need))
+
+
 class Batch:
-    """Grouped launch of many transpositions (ttc_batch_create)."""
+    """Grouped launch of many transpositions (ttc_batch_create).
+
+    The device work table holds raw pointers to every op's A and B, and a
+    captured graph replays them, so the Batch keeps a reference to each
+    tensor it was given for as long as it lives."""
 
     def __init__(self, ops, wg_per_cu=0):
         """ops: list of dicts with type_a, type_b, perm, size, A, B, alpha, beta,
-        optional lda, ldb, beta_is_zero, config."""
+        optional lda, ldb, beta_is_zero, config.  Raises ValueError when a
+        tensor is smaller than the extent its op spans."""
         L = _load()
+        for i, o in enumerate(ops):
+            ea, eb = extent_bytes(o["type_a"], o.get("type_b", o["type_a"]), o["perm"], o["size"], o.get("lda"),
+                                  o.get("ldb"))
+            _check_tensor("A", o.get("A"), ea, "Batch op %d" % i)
+            _check_tensor("B", o.get("B"), eb, "Batch op %d" % i)
+        self._keep = [(o.get("A"), o.get("B")) for o in ops]
         arr = (BatchOp * len(ops))()
         for i, o in enumerate(ops):
             b = arr[i]

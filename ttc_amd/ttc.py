@@ -316,7 +316,15 @@ def generateTransposition(ttcArgs):
                 if ret != -1:
                     best_cfg, bandwidth = ret
                     solutionFound = True
-                    if ttcArgs.silent != 1:
+                    # The key holds no lda/ldb (as the reference's, ttc/ttc.py:792-833):
+                    # a stored tile / vector width may not fit this call's padding.
+                    try:
+                        rt.Plan(ta, tb, ttcArgs.idxPerm, ttcArgs.size, lda, ldb, bz, best_cfg)
+                    except rt.TTCError:
+                        best_cfg, bandwidth, solutionFound = None, 0.0, False
+                        if ttcArgs.silent != 1:
+                            print("A stored solution does not fit these leading dimensions; searching anew.")
+                    if solutionFound and ttcArgs.silent != 1:
                         print("Solution already exists: generating the solution which was found previously.")
 
     # ------------------------------------------------------------ search
