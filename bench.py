@@ -100,9 +100,9 @@ def build_c2(torch, rt, wl, dev, tune=0):
     return ops, bufs
 
 
-def verify_sample(torch, op, nsample=4096):
-    """For a beta = 0, alpha = 1 op: B at sampled output positions equals A
-    at the matching input positions (exact)."""
+def sample_offsets(torch, op, nsample=4096):
+    """Sampled element positions of an op: (A offsets, B offsets) as device
+    index tensors (each output depends on exactly one input)."""
     import numpy as np
     rng = np.random.default_rng(op["k"] * 10 + op["es"])
     size, perm = op["size"], op["perm"]
@@ -121,22 +121,40 @@ def verify_sample(torch, op, nsample=4096):
         offB += idx[perm[q]] * m
         m *= LB[q]
     dev = op["A"].device
-    a = op["A"][torch.from_numpy(offA).to(dev)]
-    b = op["B"][torch.from_numpy(offB).to(dev)]
-    return bool(torch.equal(a, b))
+    return torch.from_numpy(offA).to(dev), torch.from_numpy(offB).to(dev)
+
+
+def verify_pass(torch, ops, stream, key):
+    """Runs every op once and checks sampled outputs against the element
+    formula (Appendix-A semantics with alpha = 1: beta = 0 gives B = A,
+    beta = 1 gives B = fl(A + B_old), one rounding in B's type).  The beta = 0
+    ops run first; the beta = 1 ops then read the B they wrote.  Returns the
+    number of ops whose sample differs."""
+    bad = 0
+    for bz in (True, False):
+        sel = [o for o in ops if o["bz"] == bz]
+        samples = [sample_offsets(torch, o) for o in sel]
+        old = [o["B"][ib].clone() for o, (_, ib) in zip(sel, samples)]
+        for o in sel:
+            o[key].execute(o["A"], o["B"], o["alpha"], o["beta"], stream=stream)
+        torch.cuda.synchronize()
+        for o, (ia, ib), b0 in zip(sel, samples, old):
+            a = o["A"][ia]
+            want = a if bz else a + b0  # alpha = 1: fl(1*a) = a, fl(1*b) = b
+            if not torch.equal(o["B"][ib], want):
+                bad += 1
+                log("  verification FAILED: k=%d %s beta=%g" % (o["k"], "f32" if o["es"] == 4 else "f64",
+                                                                 o["beta"]))
+    return bad
 
 
 def run_c2(torch, ops, stream, steps, warmup, key="plan"):
     """Times `steps` passes over the ops with one pair of events around the
     whole region; nothing else is enqueued between the launches (an event
     recorded between two kernels adds a marker packet and costs several
-    microseconds per op)."""
-    # sample check of every beta = 0 op before the beta = 1 ops reuse B
-    for o in ops:
-        if o["bz"]:
-            o[key].execute(o["A"], o["B"], o["alpha"], o["beta"], stream=stream)
-    torch.cuda.synchronize()
-    ok = all(verify_sample(torch, o) for o in ops if o["bz"])
+    microseconds per op).  Before that, every op is checked once on
+    sampled outputs (verify_pass)."""
+    ok = verify_pass(torch, ops, stream, key) == 0
     for _ in range(warmup):
         for o in ops:
             o[key].execute(o["A"], o["B"], o["alpha"], o["beta"], stream=stream)
@@ -152,6 +170,35 @@ def run_c2(torch, ops, stream, steps, warmup, key="plan"):
     torch.cuda.synchronize()
     wall = time.perf_counter() - wall0
     return t0.elapsed_time(t1), wall, ok
+
+
+def copy_ceiling(torch, rt, dev, stream, rounds=10, nbuf=8):
+    """The practical ceiling of this launch pattern: this library's identity
+    transposition (fused to one row, the row-copy kernel, beta = 0) of
+    200 MiB of fp32, launched back to back over `nbuf` distinct buffer pairs
+    as the C2 step launches its ops, so that no launch finds its data in the
+    256 MB last-level cache.  GB/s of read + write bytes."""
+    n = (200 << 20) // 4
+    bufs = [(torch.rand(n, device=dev), torch.empty(n, device=dev)) for _ in range(nbuf)]
+    plan = rt.Plan(rt.S, rt.S, [0], [n], beta_is_zero=True)
+    for A, B in bufs:
+        plan.execute(A, B, 1.0, 0.0, stream=stream)
+    torch.cuda.synchronize()
+    ok = all(torch.equal(A, B) for A, B in bufs)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(rounds):
+        for A, B in bufs:
+            plan.execute(A, B, 1.0, 0.0, stream=stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / (rounds * nbuf)
+    info = plan.info()
+    del bufs
+    torch.cuda.empty_cache()
+    return {"GBps": round(2 * n * 4 / (ms * 1e-3) / 1e9, 1), "avg_launch_ms": round(ms, 5), "bytes": 2 * n * 4,
+            "launches": rounds * nbuf, "verified": ok, "kernel": "row copy (family %d), vector %d" % (
+                info["family"], info["vec_a"])}
 
 
 def per_op_ms(torch, ops, stream, reps):
@@ -188,62 +235,172 @@ def kernel_avg_ms(torch, ops, stream, reps):
     return t0.elapsed_time(t1) / n, sum(o["bytes"] for o in ops) * reps // n
 
 
-def cpu_baseline(budget_s):
-    """The oracle (C++ restatement of TTC's reference loop nest, OpenMP on all
-    host cores) on a bounded sample of the same workload."""
-    import numpy as np
+def host_threads():
+    """CPU threads this process may use (its affinity mask; OMP_NUM_THREADS
+    when set lower)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and 0 < int(env) < n:
+        n = int(env)
+    return n
 
-    from oracle import oracle
-    from ttc_amd import workloads as wl
-    nthreads = os.cpu_count() or 1
-    cases = wl.plain_c2()
-    done, tot_bytes, tot_s = [], 0, 0.0
-    trash = np.ones((512 << 20) // 8)
-    t_start = time.perf_counter()
-    for case in [cases[i] for i in (2, 6, 11, 3, 0, 9)]:
-        if time.perf_counter() - t_start > budget_s:
-            break
-        size, perm = case["sizes"][4], case["perm"]
-        n = int(np.prod(size))
-        rng = np.random.default_rng(case["k"])
-        A = rng.uniform(-1, 1, n).astype(np.float32)
-        B = rng.uniform(-1, 1, n).astype(np.float32)
-        for bz in (True, False):
-            best = None
-            for _ in range(2):
-                oracle.trash_cache(trash, nthreads)
-                t = time.perf_counter()
-                oracle.transpose(oracle.S, oracle.S, A, B, 1.0, 0.0 if bz else 1.0, perm, size, nthreads=nthreads)
-                dt = time.perf_counter() - t
-                best = dt if best is None else min(best, dt)
-            tot_bytes += n * 4 * (2 if bz else 3)
-            tot_s += best
-            done.append("k%d/%s" % (case["k"], "b0" if bz else "b1"))
-    model = ""
+
+def l3_bytes():
+    """Total last-level cache of the host (distinct L3 instances in sysfs)."""
+    seen, total = set(), 0
+    base = "/sys/devices/system/cpu"
+    try:
+        for cpu in os.listdir(base):
+            d = os.path.join(base, cpu, "cache", "index3")
+            if not cpu.startswith("cpu") or not os.path.isdir(d):
+                continue
+            with open(os.path.join(d, "shared_cpu_list")) as f:
+                key = f.read().strip()
+            if key in seen:
+                continue
+            seen.add(key)
+            with open(os.path.join(d, "size")) as f:
+                v = f.read().strip()
+            total += int(v[:-1]) * 1024 if v.endswith("K") else int(v)
+    except (OSError, ValueError):
+        return 0
+    return total
+
+
+def cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
                 if line.startswith("model name"):
-                    model = line.split(":", 1)[1].strip()
-                    break
+                    return line.split(":", 1)[1].strip()
     except OSError:
         pass
+    return "unknown CPU"
+
+
+def cpu_oracle_rate(oracle, np, ops, nthreads, trash, reps=4):
+    """Bytes and seconds of the oracle over (tA, A, B, perm, size, beta)
+    ops: min of `reps` runs of each, each after a cache-trash sweep
+    (ttc/transposeGenerator.py:1493-1502, min of nRepeat as 452-837)."""
+    tot_bytes, tot_s = 0, 0.0
+    for t, A, B, perm, size, beta in ops:
+        best = None
+        for _ in range(reps):
+            oracle.trash_cache(trash, nthreads)
+            t0 = time.perf_counter()
+            oracle.transpose(t, t, A, B, 1.0, beta, perm, size, nthreads=nthreads)
+            dt = time.perf_counter() - t0
+            best = dt if best is None else min(best, dt)
+        n = int(np.prod(size))
+        tot_bytes += n * A.itemsize * (2 if beta == 0 else 3)
+        tot_s += best
+    return tot_bytes, tot_s
+
+
+def cpu_baseline(budget_s):
+    """The oracle (C++ restatement of TTC's reference loop nest, OpenMP on
+    the host threads this process may use) on a bounded sample of the same
+    workload: C2 cases in fp32 and fp64, beta 0 and 1, min of 4 reps each
+    after a cache-trash sweep of twice the host's L3 (at least 512 MiB),
+    until `budget_s` seconds of CPU work are spent."""
+    import numpy as np
+
+    from oracle import oracle
+    from ttc_amd import workloads as wl
+    nthreads = host_threads()
+    cases = wl.plain_c2()
+    done, tot_bytes, tot_s = [], 0, 0.0
+    trash_bytes = max(512 << 20, 2 * l3_bytes())
+    trash = np.ones(trash_bytes // 8)
+    t_start = time.perf_counter()
+    order = [(k, es) for k in (2, 6, 11, 3, 0, 9, 14, 7, 12, 5, 1, 8, 4, 10, 13, 15) for es in (4, 8)]
+    for k, es in order:
+        if time.perf_counter() - t_start > budget_s:
+            break
+        case = cases[k]
+        size, perm = case["sizes"][es], case["perm"]
+        lda = case["lda"][es] if case["lda"] else None
+        ldb = case["ldb"][es] if case["ldb"] else None
+        if lda or ldb:
+            continue  # the padded case: dense extents only here
+        n = int(np.prod(size))
+        rng = np.random.default_rng(k)
+        dt = np.float32 if es == 4 else np.float64
+        A = rng.random(n, dtype=dt)
+        B = rng.random(n, dtype=dt)
+        t = oracle.S if es == 4 else oracle.D
+        b, s_ = cpu_oracle_rate(oracle, np, [(t, A, B, perm, size, 0.0), (t, A, B, perm, size, 1.0)], nthreads,
+                                trash)
+        tot_bytes += b
+        tot_s += s_
+        done.append("k%d/%s" % (k, "f32" if es == 4 else "f64"))
+        del A, B
     return {"value": round(tot_bytes / tot_s / 1e9, 3), "unit": "GB/s", "cores": nthreads, "kind": "port",
-            "sample": "oracle loop nest (OpenMP, %d threads, %s) on fp32 C2 cases %s, min of 2 reps each after a "
-                      "512 MiB cache-trash sweep" % (nthreads, model, ",".join(done))}
+            "sample": "oracle loop nest (OpenMP, %d threads, %s) on C2 cases %s, beta 0 and 1 each, min of 4 reps "
+                      "after a %d MiB cache-trash sweep (2x L3)" % (nthreads, cpu_model(), ",".join(done),
+                                                                  trash_bytes >> 20)}
+
+
+def run_c1(torch, rt, wl, dev, stream, reps=20, nbuf=4):
+    """The plain CPU-runnable case at full size (fp32, perm (1,0,2), size
+    (1000,768,16), alpha = beta = 1): GPU rate over launches alternating
+    between `nbuf` buffer pairs, and the oracle's rate on the host."""
+    import numpy as np
+
+    from oracle import oracle
+    c1 = wl.plain_c1()
+    perm, size = c1["perm"], c1["size"]
+    n = int(np.prod(size))
+    plan = rt.Plan(rt.S, rt.S, perm, size, beta_is_zero=False)
+    nb = plan.info()["bytes_algorithmic"]
+    bufs = [(torch.rand(n, device=dev) * 2 - 1, torch.rand(n, device=dev) * 2 - 1) for _ in range(nbuf)]
+    for A, B in bufs:
+        plan.execute(A, B, c1["alpha"], c1["beta"], stream=stream)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for r in range(reps):
+        A, B = bufs[r % nbuf]
+        plan.execute(A, B, c1["alpha"], c1["beta"], stream=stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del bufs
+    nthreads = host_threads()
+    rng = np.random.default_rng(0)
+    hA = rng.uniform(-1, 1, n).astype(np.float32)
+    hB = rng.uniform(-1, 1, n).astype(np.float32)
+    trash = np.ones(max(512 << 20, 2 * l3_bytes()) // 8)
+    b, s_ = cpu_oracle_rate(oracle, np, [(oracle.S, hA, hB, perm, size, 1.0)], nthreads, trash)
+    gpu = nb / (ms * 1e-3) / 1e9
+    cpu = b / s_ / 1e9
+    return {"ms": round(ms, 4), "GBps": round(gpu, 1), "frac": round(gpu / HBM_PEAK_GBS, 4),
+            "tile": plan.info()["tile"], "cpu_oracle_GBps": round(cpu, 2), "cpu_threads": nthreads,
+            "gpu_over_cpu": round(gpu / cpu, 1)}
 
 
 def run_batch(torch, rt, wl, dev, stream, replays, tune=0):
-    """512 mixed ops: grouped persistent launch (graph replay) vs the per-op
-    launch loop with the same kernels.  tune > 0: every op of at least 1 MiB
-    takes the autotuner's configuration (at most `tune` candidates); the
-    planner's choices are timed beside it."""
+    """512 mixed ops: the grouped persistent launch (graph replay) against
+    plain per-op launch loops over the same ops and buffers.
+
+    The plain loops are what a user writes with ttc_plan_create /
+    ttc_execute: one plan per op with its own best single-op configuration
+    (the planner's, or the autotuner's when tuning: every kernel family,
+    row copy and two LDS buffers included), launched one after another on
+    one stream — issued from Python, and captured once into a graph and
+    replayed (no host time per launch).  The speed-up is quoted against the
+    fastest of them.  Diagnostics: the loop over the batch's own plans
+    (grouped kinds only) and one synchronous legacy call per op.
+    tune > 0: every op of at least 1 MiB takes the autotuner's configuration
+    (at most `tune` candidates) in the batch and in the plain loops."""
     ops = wl.batch_c5()
     gen = torch.Generator(device=dev)
     gen.manual_seed(5)
     tmap = {0: (rt.S, torch.float32), 1: (rt.D, torch.float64), 4: (rt.BF16, torch.bfloat16)}
     desc = []
-    keep = []
     for o in ops:
         t, tdt = tmap[o["type"]]
         n = 1
@@ -251,7 +408,6 @@ def run_batch(torch, rt, wl, dev, stream, replays, tune=0):
             n *= x
         A = (torch.rand(n, generator=gen, device=dev, dtype=torch.float32) * 2 - 1).to(tdt)
         B = (torch.rand(n, generator=gen, device=dev, dtype=torch.float32) * 2 - 1).to(tdt)
-        keep.append((A, B))
         desc.append({"type_a": t, "type_b": t, "perm": o["perm"], "size": o["size"], "A": A, "B": B,
                      "alpha": o["alpha"], "beta": o["beta"]})
     batch = rt.Batch(desc)
@@ -268,6 +424,30 @@ def run_batch(torch, rt, wl, dev, stream, replays, tune=0):
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / reps
 
+    def plain_loops(cfgs):
+        """(eager ms, graph-replay ms) of the per-op plan loop."""
+        plans = [rt.Plan(d["type_a"], d["type_b"], d["perm"], d["size"], beta_is_zero=d["beta"] == 0, config=c)
+                 for d, c in zip(desc, cfgs)]
+
+        def loop():
+            for p, d in zip(plans, desc):
+                p.execute(d["A"], d["B"], d["alpha"], d["beta"], stream=stream)
+        t_eager = timed(loop, max(5, replays // 100))
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.stream(stream):
+            loop()
+            stream.synchronize()
+            with torch.cuda.graph(g, stream=stream):
+                loop()
+
+        def replay():
+            with torch.cuda.stream(stream):
+                g.replay()
+        t_graph = timed(replay, max(10, replays // 10))
+        del g
+        return t_eager, t_graph
+
+    cfgs = [None] * len(desc)
     t_group = timed(lambda: batch.graph_launch(stream, per_op_loop=False), replays)
     t_group_planner, tuned = None, 0
     if tune > 0:
@@ -282,8 +462,13 @@ def run_batch(torch, rt, wl, dev, stream, replays, tune=0):
         else:
             tb.close()
     t_group_eager = timed(lambda: batch.execute(stream), max(10, replays // 10))
-    t_loop = timed(lambda: batch.execute_loop(stream), max(10, replays // 10))
-    t_loop_graph = timed(lambda: batch.graph_launch(stream, per_op_loop=True), max(10, replays // 10))
+    loops = {"planner": plain_loops([None] * len(desc))}
+    if tune > 0:
+        loops["tuned"] = plain_loops(cfgs)
+    best_name, best_ms = min(((k + "_" + m, v[i]) for k, v in loops.items() for i, m in enumerate(("eager", "graph"))),
+                             key=lambda x: x[1])
+    t_loop_batch = timed(lambda: batch.execute_loop(stream), max(10, replays // 10))
+    t_loop_batch_graph = timed(lambda: batch.graph_launch(stream, per_op_loop=True), max(10, replays // 10))
 
     # the reference wrapper's semantics: one synchronous call per op on the
     # null stream (ttc/CUDAtranspose.py:395-429), here without its per-call
@@ -300,21 +485,25 @@ def run_batch(torch, rt, wl, dev, stream, replays, tune=0):
         legacy_loop()
     t_sync = (time.perf_counter() - w0) * 1e3 / reps
     n = len(ops)
-    return {"ops": n, "replays": replays, "tiles": info["tiles"], "workgroups": info["workgroups"],
-            "bytes_per_launch": info["bytes"],
-            "configs": ("autotuned (<= %d candidates) for the %d distinct problems of >= 1 MiB, planner for the "
-                        "rest" % (tune, tuned)) if tune else "planner",
-            "grouped_graph_ms": round(t_group, 4),
-            "grouped_graph_ms_planner": round(t_group_planner, 4) if t_group_planner else None,
-            "grouped_eager_ms": round(t_group_eager, 4),
-            "loop_async_ms": round(t_loop, 4), "loop_graph_ms": round(t_loop_graph, 4),
-            "loop_sync_legacy_ms": round(t_sync, 3),
-            "transposes_per_s_grouped": round(n / (t_group * 1e-3), 1),
-            "transposes_per_s_loop": round(n / (t_loop * 1e-3), 1),
-            "speedup_vs_async_loop": round(t_loop / t_group, 3),
-            "speedup_vs_graph_loop": round(t_loop_graph / t_group, 3),
-            "speedup_vs_sync_legacy_loop": round(t_sync / t_group, 3),
-            "grouped_GBps": round(info["bytes"] / (t_group * 1e-3) / 1e9, 1)}
+    out = {"ops": n, "replays": replays, "tiles": info["tiles"], "workgroups": info["workgroups"],
+           "bytes_per_launch": info["bytes"],
+           "configs": ("autotuned (<= %d candidates) for the %d distinct problems of >= 1 MiB, planner for the "
+                       "rest" % (tune, tuned)) if tune else "planner",
+           "grouped_graph_ms": round(t_group, 4),
+           "grouped_graph_ms_planner": round(t_group_planner, 4) if t_group_planner else None,
+           "grouped_eager_ms": round(t_group_eager, 4),
+           "plain_loop_ms": {k + "_" + m: round(v[i], 4) for k, v in loops.items()
+                             for i, m in enumerate(("eager", "graph"))},
+           "best_plain_loop": best_name,
+           "transposes_per_s_grouped": round(n / (t_group * 1e-3), 1),
+           "transposes_per_s_best_plain_loop": round(n / (best_ms * 1e-3), 1),
+           "speedup_vs_best_plain_loop": round(best_ms / t_group, 3),
+           "grouped_GBps": round(info["bytes"] / (t_group * 1e-3) / 1e9, 1),
+           "diagnostic": {"loop_batch_plans_ms": round(t_loop_batch, 4),
+                          "loop_batch_plans_graph_ms": round(t_loop_batch_graph, 4),
+                          "loop_sync_legacy_ms": round(t_sync, 3),
+                          "speedup_vs_sync_legacy_loop": round(t_sync / t_group, 3)}}
+    return out
 
 
 def run_workloads(torch, rt, wl, dev, stream, reps=5, tune=0):
@@ -426,7 +615,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=25.0)
     ap.add_argument("--batch", dest="batch", action="store_true", default=True)
     ap.add_argument("--no-batch", dest="batch", action="store_false")
     ap.add_argument("--batch-replays", type=int, default=1000)
@@ -460,13 +649,17 @@ def main():
         ops, bufs = build_c2(torch, rt, wl, dev, args.tune)
     torch.cuda.synchronize()
     log("c2: %d ops per step" % len(ops))
-    value_planner = None
+    value_planner, ok0 = None, True
     if args.tune > 0:  # the planner's own choice, for comparison
-        t0_ms, _, _ = run_c2(torch, ops, stream, args.steps, args.warmup, key="plan0")
+        t0_ms, _, ok0 = run_c2(torch, ops, stream, args.steps, args.warmup, key="plan0")
         value_planner = sum(o["bytes"] for o in ops) * args.steps / (t0_ms * 1e-3) / 1e9
     total_ms, wall, ok = run_c2(torch, ops, stream, args.steps, args.warmup)
+    ok = ok and ok0
     step_bytes = sum(o["bytes"] for o in ops)
     value = step_bytes * args.steps / (total_ms * 1e-3) / 1e9
+    ceiling = copy_ceiling(torch, rt, dev, stream)
+    log("  copy ceiling (row copy, 200 MiB fp32, %d launches over 8 buffer pairs): %.1f GB/s" % (
+        ceiling["launches"], ceiling["GBps"]))
     # per-op and per-kernel times (diagnostic pass, each launch between events)
     per_op = per_op_ms(torch, ops, stream, 5)
     groups = {}
@@ -495,7 +688,9 @@ def main():
         dom, len(groups[dom]["ops"]), max(2, args.steps), avg_ms, achieved))
     out = {
         "metric": "GB/s (r+w) & % of HBM peak per transpose; batched transposes/sec vs plain",
-        "value": round(value, 2), "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        # a step whose sampled outputs differ from the element formula has no value
+        "value": round(value, 2) if ok else None, "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup,
         "ms_per_step": round(total_ms / args.steps, 4), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32,f64", "data": "synthetic",
         "config": {"workload": "16 seeded random perms rank 2-6, 200 MiB A each, fp32+fp64, beta 0 and 1 "
@@ -505,8 +700,12 @@ def main():
                    "value_GiBps": round(value * 1e9 / 2 ** 30, 2),
                    "pct_of_hbm_peak": round(100.0 * value / HBM_PEAK_GBS, 2), "verified_sample": ok,
                    "timed_wall_s": round(wall, 3)},
+        "copy_ceiling_GBps": ceiling["GBps"],
+        "frac_of_copy": round(value / ceiling["GBps"], 4),
+        "copy_ceiling": ceiling,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom),
+                     "frac_of_copy": round(achieved / ceiling["GBps"], 4),
                      "avg_launch_ms": round(avg_ms, 5), "bytes_per_launch": bytes_per_launch,
                      "launches": len(groups[dom]["ops"]) * max(2, args.steps)},
     }
@@ -519,11 +718,15 @@ def main():
             out["batched"] = {"error": str(e)}
     if args.workloads:
         try:
-            out["workloads"] = run_workloads(torch, rt, wl, dev, stream, tune=min(args.tune, 8))
+            out["workloads"] = {"plain_c1": run_c1(torch, rt, wl, dev, stream)}
+            out["workloads"].update(run_workloads(torch, rt, wl, dev, stream, tune=min(args.tune, 8)))
         except Exception as e:  # reported, not hidden
-            out["workloads"] = {"error": str(e)}
+            out.setdefault("workloads", {})["error"] = str(e)
     out["cpu_baseline"] = None if args.no_cpu else cpu_baseline(args.cpu_seconds)
     print(json.dumps(out), flush=True)
+    if not ok:
+        log("bench.py: sampled outputs differ from the element formula; no value reported")
+        sys.exit(1)
 
 
 if __name__ == "__main__":

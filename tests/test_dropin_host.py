@@ -137,3 +137,12 @@ def test_library_reports_the_digest_of_its_sources():
     v = rt.version()
     assert v.startswith("ttc-mi355x %s (gfx950) src:" % hb.VERSION) and len(v.split("src:")[1]) == 16
     assert v == hb.version_string(), "libttc_hip.so is stale: rebuild with python -m ttc_amd.build"
+
+
+def test_throughput_mode_rejects_what_it_cannot_check():
+    from ttc_amd import runtime as rt
+    from ttc_amd import throughput
+    with pytest.raises(ValueError):
+        throughput.run_streams(rt.C, rt.C, [1, 0], [4, 4])
+    with pytest.raises(ValueError):
+        throughput.run_processes(rt.S, rt.S, [1, 0], [4, 4], processes=17)

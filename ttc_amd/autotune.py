@@ -13,6 +13,7 @@ prebuilt kernel with different arguments, timed with HIP events.
 import math
 
 from . import runtime as rt
+from . import ttc_util
 
 
 def tile_score(info, size_a, size_b):
@@ -36,12 +37,63 @@ def tile_score(info, size_a, size_b):
     return sc
 
 
+def _row_group(info):
+    """A's leading dimensions that form a row x column tile's rows: from
+    dimension 0 up to B's stride-1 dimension, each covered by the tile
+    (csrc/ttc_plan.cpp rc_groups)."""
+    r, perm, size, tile = info["rank"], info["perm"], info["size"], info["tile"]
+    rows = []
+    for k in range(r):
+        if k == perm[0] or (k > 0 and (tile[k] == 1 or size[k - 1] % tile[k - 1])):
+            break
+        rows.append(k)
+    return rows
+
+
+def tile_walk_cost(info, tile_order):
+    """TTC's loop-order cost (ttc_util.walkCost, ttc/ttc_util.py:167-209)
+    of the order in which a row x column plan numbers its tiles: the tile
+    grid is a loop nest whose loop over dimension k has ceil(n_k / t_k)
+    iterations and steps t_k elements in A and in B.  tile_order 1 walks A
+    order (dimension 0 innermost), 2 B order, 3 / 4 B order after an inner
+    group of 8 / 16 tiles along the rows' outermost tiled dimension."""
+    r, perm, size, tile = info["rank"], info["perm"], info["size"], info["tile"]
+    sa, sb = info["stride_a"], info["stride_b"]
+    g = [-(-size[k] // tile[k]) if size[k] else 1 for k in range(r)]
+    order = list(range(r)) if tile_order == 1 else list(perm)  # innermost first
+    inner_first = [(g[k], tile[k] * sa[k], tile[k] * sb[k]) for k in order if g[k] > 1]
+    if tile_order in (3, 4):
+        G = 8 if tile_order == 3 else 16
+        kg = next((k for k in reversed(_row_group(info)) if g[k] > 1), None)
+        if kg is not None:
+            gi = min(G, g[kg])
+            outer = -(-g[kg] // gi)
+            inner_first = [(gi, tile[kg] * sa[kg], tile[kg] * sb[kg])]
+            for k in order:
+                if k == kg:
+                    if outer > 1:
+                        inner_first.append((outer, gi * tile[k] * sa[k], gi * tile[k] * sb[k]))
+                elif g[k] > 1:
+                    inner_first.append((g[k], tile[k] * sa[k], tile[k] * sb[k]))
+    return ttc_util.walkCost(list(reversed(inner_first)))
+
+
 def candidates(type_a, type_b, perm, size, lda=None, ldb=None, beta_is_zero=False, max_impl=200,
-               nt_stores=False):
+               nt_stores=False, blockings=None):
     """List of config dicts, best-first by the model, at most max_impl.  The
-    planner's own choice ({}) always comes first.  Non-temporal stores are
-    tried only with nt_stores: they measure well in a loop over a few
-    buffers and lose in longer streams of different transpositions."""
+    planner's own choice ({}) comes first unless `blockings` is given.
+
+    nt_stores: also try non-temporal B stores (beta == 0 only; the command
+    line's --streamingStores).  They measure well in a loop over a few
+    buffers and lose in longer streams of different transpositions, so they
+    are not tried by default.
+    blockings: [(a, b), ...] restricts the tiles to those derived from these
+    run-length targets on A's and B's side, as the reference restricts its
+    blockings (--blockings, ttc/ttc.py:1296-1297, ttc/GPUtransposeGenerator.py:
+    144-150).
+    Tile walks of row x column tiles are tried in the order of TTC's
+    loop-order cost of their tile grid (tile_walk_cost), as the reference
+    sorts its loop orders by getCostLoop before pruning."""
     base = rt.Plan(type_a, type_b, perm, size, lda, ldb, beta_is_zero).info()
     sa, sb = rt.DTYPE_BYTES[type_a], rt.DTYPE_BYTES[type_b]
     vecs = []
@@ -51,74 +103,78 @@ def candidates(type_a, type_b, perm, size, lda=None, ldb=None, beta_is_zero=Fals
                 vecs.append((va, vb))
     tiles = {}
     runs = [1 << k for k in range(0, 14)]
+    targets = list(blockings) if blockings else [(ra, rb) for ra in runs for rb in runs if ra * rb <= 4 * 16384]
     for fam in (2, 1):
         for va, vb in vecs:
-            for ra in runs:
-                for rb in runs:
-                    if ra * rb > 4 * 16384:
-                        continue
-                    cfg = {"vec_a": va, "vec_b": vb, "run_a": ra, "run_b": rb, "family": fam}
-                    try:
-                        info = rt.Plan(type_a, type_b, perm, size, lda, ldb, beta_is_zero, cfg).info()
-                    except rt.TTCError:
-                        continue
-                    key = (fam, tuple(info["tile"]), va, vb)
-                    if key not in tiles:
-                        tiles[key] = (tile_score(info, sa, sb),
-                                      {"tile": info["tile"], "vec_a": va, "vec_b": vb, "family": fam})
-    ranked = [c for _, c in sorted(tiles.values(), key=lambda x: -x[0])]
+            for ra, rb in targets:
+                cfg = {"vec_a": va, "vec_b": vb, "run_a": int(ra), "run_b": int(rb), "family": fam}
+                try:
+                    info = rt.Plan(type_a, type_b, perm, size, lda, ldb, beta_is_zero, cfg).info()
+                except rt.TTCError:
+                    continue
+                key = (fam, tuple(info["tile"]), va, vb)
+                if key not in tiles:
+                    tiles[key] = (tile_score(info, sa, sb),
+                                  {"tile": info["tile"], "vec_a": va, "vec_b": vb, "family": fam}, info)
+    ranked = [(c, i) for _, c, i in sorted(tiles.values(), key=lambda x: -x[0])]
     # distinct tile shapes first (each with its best-scoring widths), then the
     # other widths of shapes already listed, so that pruning keeps shapes
     seen, first, rest = set(), [], []
-    for c in ranked:
+    for c, i in ranked:
         key = (c["family"], tuple(c["tile"]))
-        (rest if key in seen else first).append(c)
+        (rest if key in seen else first).append((c, i))
         seen.add(key)
     ranked = first + rest
     # the best tile of each family first, so that pruning never drops a family
     heads = []
     for fam in (2, 1):
-        best = next((c for c in ranked if c["family"] == fam), None)
+        best = next((x for x in ranked if x[0]["family"] == fam), None)
         if best is not None:
             heads.append(best)
-    ranked = heads + [c for c in ranked if all(c is not h for h in heads)]
-    # launch variants: tile numbering (row x column tiles), tile walk, store mode
-    launch = []
-    for order in (4, 3, 1):
-        for tord in (2, 3, 1):  # B order, B order grouped by 8 along A's rows, A order
-            for nt in ((2, 1) if beta_is_zero and nt_stores else (0,)):
-                launch.append({"grid_order": order, "tile_order": tord, "nt_store": nt})
+    ranked = heads + [x for x in ranked if all(x is not h for h in heads)]
+    # launch variants: tile walk (grid order), tile numbering (row x column
+    # tiles, per tile in loop-order-cost order), store mode
+    grid_orders = (4, 3, 1)
+    nts = (2, 1) if beta_is_zero and nt_stores else (0,)
+    n_launch = len(grid_orders) * 3 * len(nts)
     # sqrt pruning of both axes, as the reference combines its best blockings
     # with its best loop orders
-    budget = max(1, max_impl - 1)
+    budget = max(1, max_impl - (0 if blockings else 1))
     k = max(1, int(math.ceil(math.sqrt(budget))))
-    while len(ranked) * len(launch) > budget and len(ranked) > k:
+    while len(ranked) * n_launch > budget and len(ranked) > k:
         ranked.pop()
-    while len(ranked) * len(launch) > budget and len(launch) > 1:
-        launch.pop()
-    out = [{}]
-    if base["family"] == 3:  # row copy (perm[0] == 0): resident work-groups per CU
+    while len(ranked) * n_launch > budget and n_launch > 1:
+        n_launch -= 1
+    out = [] if blockings else [{}]
+    if base["family"] == 3 and not blockings:  # row copy (perm[0] == 0): resident work-groups per CU
         out += [{"family": 3, "wg_per_cu": w} for w in (4, 2)]
-    for t in ranked:
+    for t, info in ranked:
+        if t["family"] == 2:
+            tords = sorted((2, 3, 1), key=lambda o: (tile_walk_cost(info, o), o))
+        else:
+            tords = (2,)  # box tiles have one numbering
+        launch = [{"grid_order": go, "tile_order": to, "nt_store": nt}
+                  for go in grid_orders for to in tords for nt in nts][:n_launch]
         for l in launch:
-            if t["family"] == 1 and l["tile_order"] != 2:
-                continue  # box tiles have one numbering
             c = dict(t)
             c.update(l)
             out.append(c)
     # waves per CU: the best tile with fewer resident work-groups than the
     # occupancy limit (the default), where the budget leaves room
     if ranked:
+        t0, i0 = ranked[0]
+        l0 = {"grid_order": grid_orders[0], "nt_store": nts[0],
+              "tile_order": min((2, 3, 1), key=lambda o: (tile_walk_cost(i0, o), o)) if t0["family"] == 2 else 2}
         for w in (2, 3):
             if len(out) < max_impl:
-                c = dict(ranked[0])
-                c.update(launch[0])
+                c = dict(t0)
+                c.update(l0)
                 c["wg_per_cu"] = w
                 out.append(c)
         # one LDS buffer (two barriers per tile, more resident work-groups)
-        if len(out) < max_impl and ranked[0]["family"] == 2:
-            c = dict(ranked[0])
-            c.update(launch[0])
+        if len(out) < max_impl and t0["family"] == 2:
+            c = dict(t0)
+            c.update(l0)
             c["lds_buffers"] = 1
             out.append(c)
     return out[:max(1, max_impl)]
@@ -168,7 +224,7 @@ def _device_inputs(torch, type_a, type_b, perm, size, lda, ldb, dev):
 
 
 def tune(type_a, type_b, perm, size, lda=None, ldb=None, alpha=1.0, beta=0.0, max_impl=200, reps=5,
-         progress=None, nt_stores=False, hot_a=False, hot_b=False):
+         progress=None, nt_stores=False, hot_a=False, hot_b=False, blockings=None):
     """Times every candidate on device buffers of the problem's shape.
     Returns results sorted fastest first: dicts with config, info, ms, GBs
     (10^9 B/s) and GiBs (2^30 B/s, the reference's unit).  hot_a / hot_b:
@@ -190,7 +246,7 @@ def tune(type_a, type_b, perm, size, lda=None, ldb=None, alpha=1.0, beta=0.0, ma
             A2, B2 = _device_inputs(torch, type_a, type_b, perm, size, lda, ldb, dev)
             pairs.append((A if hot_a else A2, B if hot_b else B2))
     results = []
-    cands = candidates(type_a, type_b, perm, size, lda, ldb, bz, max_impl, nt_stores)
+    cands = candidates(type_a, type_b, perm, size, lda, ldb, bz, max_impl, nt_stores, blockings)
     for i, cfg in enumerate(cands):
         try:
             plan = rt.Plan(type_a, type_b, perm, size, lda, ldb, bz, cfg)

@@ -17,8 +17,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <new>
+#include <type_traits>
 #include <vector>
 
 #include "ttc_launch.hpp"
@@ -81,40 +83,23 @@ using KRCOp = const __attribute__((address_space(4))) BatchRCOpDev;
 __device__ __forceinline__ int32_t uni(int32_t x) { return __builtin_amdgcn_readfirstlane(x); }
 __device__ __forceinline__ uint32_t uni(uint32_t x) { return (uint32_t)__builtin_amdgcn_readfirstlane((int32_t)x); }
 
-// The parameters the per-thread setup and the tile loop read, copied out of
-// the op record with each value marked wave-uniform, so that they live in
-// scalar registers as kernel arguments do in the single-op kernel.
+// Parameter records are copied out of the work table whole, one 32-bit word
+// at a time, so that no field can be left behind when a field is added (a
+// field-by-field copy once missed one, and a kernel then read an
+// uninitialised value).  The compiler drops the words a kernel never reads.
+static_assert(sizeof(TileParams) % 4 == 0 && std::is_trivially_copyable<TileParams>::value,
+              "TileParams is copied as 32-bit words");
+static_assert(sizeof(RCParams) % 4 == 0 && std::is_trivially_copyable<RCParams>::value,
+              "RCParams is copied as 32-bit words");
+
+// A box op's parameters, each word marked wave-uniform (readfirstlane), so
+// that they live in scalar registers as kernel arguments do in the
+// single-op kernel.
 __device__ __forceinline__ void uniform_params(const TileParams& src, TileParams& p) {
-  p.rank = uni(src.rank);
-  p.V = uni(src.V);
-  p.split0 = uni(src.split0);
-  p.split1 = uni(src.split1);
-  p.tsplit0 = uni(src.tsplit0);
-  p.tsplit1 = uni(src.tsplit1);
-  p.splitB0 = uni(src.splitB0);
-  p.splitB1 = uni(src.splitB1);
-  p.lds_stride_b = uni(src.lds_stride_b);
+  const uint32_t* s = reinterpret_cast<const uint32_t*>(&src);
+  uint32_t* d = reinterpret_cast<uint32_t*>(&p);
 #pragma unroll
-  for (int k = 0; k < kMaxDim; ++k) {
-    p.t[k] = uni(src.t[k]);
-    p.sa[k] = uni(src.sa[k]);
-    p.ls[k] = uni(src.ls[k]);
-    p.tdiv[k].m = uni(src.tdiv[k].m);
-    p.tdiv[k].s = uni(src.tdiv[k].s);
-    p.tB[k] = uni(src.tB[k]);
-    p.sbB[k] = uni(src.sbB[k]);
-    p.lsB[k] = uni(src.lsB[k]);
-    p.tdivB[k].m = uni(src.tdivB[k].m);
-    p.tdivB[k].s = uni(src.tdivB[k].s);
-  }
-  const FastDiv* sd[6] = {&src.pA0, &src.pA1, &src.pB0, &src.pB1, &src.ts0, &src.ts1};
-  FastDiv* dd[6] = {&p.pA0, &p.pA1, &p.pB0, &p.pB1, &p.ts0, &p.ts1};
-#pragma unroll
-  for (int i = 0; i < 6; ++i) {
-    dd[i]->d = uni(sd[i]->d);
-    dd[i]->m = uni(sd[i]->m);
-    dd[i]->s = uni(sd[i]->s);
-  }
+  for (int i = 0; i < (int)(sizeof(TileParams) / 4); ++i) d[i] = (uint32_t)__builtin_amdgcn_readfirstlane((int32_t)s[i]);
 }
 
 template <class OP, int VA, int VB, bool BETA>
@@ -156,23 +141,14 @@ __global__ void __launch_bounds__(kThreads) batch_group_kernel(const BatchOpDev*
   }
 }
 
-// The scalar shape parameters the row x column walks read (scalar loads
-// through the constant address space).
+// A row x column op's parameters, read through the constant address space
+// (scalar loads), word by word.
+using KWord = const __attribute__((address_space(4))) uint32_t;
 __device__ __forceinline__ void scalar_rc(KRCOp& o, RCParams& p) {
-  p.R = o.p.R;
-  p.C = o.p.C;
-  p.Rl = o.p.Rl;
-  p.Cl = o.p.Cl;
-  p.dR.d = o.p.dR.d; p.dR.m = o.p.dR.m; p.dR.s = o.p.dR.s;
-  p.dRl.d = o.p.dRl.d; p.dRl.m = o.p.dRl.m; p.dRl.s = o.p.dRl.s;
-  p.dC.d = o.p.dC.d; p.dC.m = o.p.dC.m; p.dC.s = o.p.dC.s;
-  p.dCl.d = o.p.dCl.d; p.dCl.m = o.p.dCl.m; p.dCl.s = o.p.dCl.s;
-  p.RP = o.p.RP;
-  p.rowA_dense = o.p.rowA_dense;
-  p.colB_dense = o.p.colB_dense;
-  p.lds_data_elems = o.p.lds_data_elems;
-  p.lds_spare = o.p.lds_spare;
-  p.nbuf = o.p.nbuf;  // places the offset tables in LDS (setup_tables)
+  KWord* s = (KWord*)&o.p;
+  uint32_t* d = reinterpret_cast<uint32_t*>(&p);
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(RCParams) / 4); ++i) d[i] = s[i];
 }
 
 template <class OP, int VA, int VB, bool BETA>
@@ -281,37 +257,89 @@ using namespace ttc;
 // One launch of the grouped execution: the ops of one kind.
 struct BatchGroup {
   int kind;
-  int op0, nops;   // slice of the device op table of the kind's family
-  int wg0, nwg;    // slice of the device work-group table (nwg + 1 entries)
+  int op0, nops;   // slice of the op table of the kind's family
+  int wg0, nwg;    // slice of the work-group table (nwg + 1 entries)
   int lds_bytes;
 };
 
-struct ttc_batch_s {
-  int nops;
+// Host copy of a batch's device work table.  Its sections (box op records,
+// row x column op records, work-group slices, box tile records, row x
+// column tile records, offset tables) sit 256-byte aligned in one image, so
+// that one allocation (or the fixed workspace) and one copy hold all of it;
+// op records point at their tile records once the image's device address is
+// known (render).
+struct BatchImage {
+  enum { kBox, kRC, kWg, kBases, kRcTiles, kRcTables, kSections };
+  std::vector<BatchOpDev> box;
+  std::vector<BatchRCOpDev> rcs;
+  std::vector<uint32_t> wg;
+  std::vector<TileBase> bases;
+  std::vector<RCTile> rct;
+  std::vector<int32_t> rtab;
+  std::vector<size_t> box_rec;          // per box op: index of its first tile record
+  std::vector<size_t> rc_rec, rc_tab;   // per row x column op: first tile record, offset tables
+  size_t off[kSections] = {0};
+  size_t bytes = 0;
+
+  void layout() {
+    const size_t len[kSections] = {sizeof(BatchOpDev) * box.size(), sizeof(BatchRCOpDev) * rcs.size(),
+                                   sizeof(uint32_t) * wg.size(),      sizeof(TileBase) * bases.size(),
+                                   sizeof(RCTile) * rct.size(),       sizeof(int32_t) * rtab.size()};
+    size_t o = 0;
+    for (int s = 0; s < kSections; ++s) {
+      off[s] = o;
+      o += (len[s] + 255) / 256 * 256;
+    }
+    bytes = o;
+  }
+
+  // The image as it must sit at device address `base`.
+  void render(uintptr_t base, std::vector<unsigned char>& out) const {
+    out.assign(bytes, 0);
+    std::vector<BatchOpDev> b2(box);
+    for (size_t i = 0; i < b2.size(); ++i)
+      b2[i].bases = reinterpret_cast<const TileBase*>(base + off[kBases] + box_rec[i] * sizeof(TileBase));
+    std::vector<BatchRCOpDev> r2(rcs);
+    for (size_t i = 0; i < r2.size(); ++i) {
+      r2[i].tiles = reinterpret_cast<const RCTile*>(base + off[kRcTiles] + rc_rec[i] * sizeof(RCTile));
+      r2[i].tables = reinterpret_cast<const int32_t*>(base + off[kRcTables] + rc_tab[i] * sizeof(int32_t));
+    }
+    auto put = [&](int s, const void* src, size_t n) {
+      if (n) std::memcpy(out.data() + off[s], src, n);
+    };
+    put(kBox, b2.data(), sizeof(BatchOpDev) * b2.size());
+    put(kRC, r2.data(), sizeof(BatchRCOpDev) * r2.size());
+    put(kWg, wg.data(), sizeof(uint32_t) * wg.size());
+    put(kBases, bases.data(), sizeof(TileBase) * bases.size());
+    put(kRcTiles, rct.data(), sizeof(RCTile) * rct.size());
+    put(kRcTables, rtab.data(), sizeof(int32_t) * rtab.size());
+  }
+};
+
+// Everything a batch decides on the host: per-op plans, groups, the image.
+struct BatchPlan {
   std::vector<ttc_plan_t> plans;  // per-op plans (also used by the per-op loop)
   std::vector<ttc_batch_op_t> ops;
   std::vector<BatchGroup> groups;
-  BatchOpDev* d_ops;       // box ops
-  BatchRCOpDev* d_rc_ops;  // row x column ops
-  uint32_t* d_wg;
-  TileBase* d_bases;
-  RCTile* d_rc_tiles;
-  int32_t* d_rc_tables;
-  int nwg_total;
-  int64_t tiles, bytes;
-  hipGraph_t graph[2];
-  hipGraphExec_t exec[2];
+  BatchImage img;
+  int nwg_total = 0;
+  int64_t tiles = 0, bytes = 0;
+
+  ~BatchPlan() {
+    for (ttc_plan_t p : plans)
+      if (p) ttc_plan_destroy(p);
+  }
+};
+
+struct ttc_batch_s {
+  BatchPlan bp;
+  void* d_image = nullptr;  // the work table on the device
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
 };
 
 static void batch_free(ttc_batch_s* b) {
-  for (ttc_plan_t p : b->plans)
-    if (p) ttc_plan_destroy(p);
-  if (b->d_ops) (void)hipFree(b->d_ops);
-  if (b->d_rc_ops) (void)hipFree(b->d_rc_ops);
-  if (b->d_wg) (void)hipFree(b->d_wg);
-  if (b->d_bases) (void)hipFree(b->d_bases);
-  if (b->d_rc_tiles) (void)hipFree(b->d_rc_tiles);
-  if (b->d_rc_tables) (void)hipFree(b->d_rc_tables);
+  if (b->d_image) (void)hipFree(b->d_image);
   for (int i = 0; i < 2; ++i) {
     if (b->exec[i]) (void)hipGraphExecDestroy(b->exec[i]);
     if (b->graph[i]) (void)hipGraphDestroy(b->graph[i]);
@@ -405,27 +433,18 @@ static std::vector<uint32_t> split_by_bytes(const std::vector<OpRec>& h, int op0
   return wgb;
 }
 
-extern "C" ttc_status_t ttc_batch_create(ttc_batch_t* out, int nops, const ttc_batch_op_t* ops, int wg_per_cu) {
-  if (!out || nops < 1 || !ops || wg_per_cu < 0) return TTC_ERROR_INVALID_VALUE;
-  *out = nullptr;
-  ttc_batch_s* b = new (std::nothrow) ttc_batch_s();
-  if (!b) return TTC_ERROR_ALLOC;
-  b->nops = nops;
-  b->ops.assign(ops, ops + nops);
-  b->plans.assign(nops, nullptr);
+// Plans every op and builds the work table on the host (no device memory).
+static ttc_status_t plan_batch(int nops, const ttc_batch_op_t* ops, int wg_per_cu, BatchPlan& bp) {
+  if (nops < 1 || !ops || wg_per_cu < 0) return TTC_ERROR_INVALID_VALUE;
+  bp.ops.assign(ops, ops + nops);
+  bp.plans.assign(nops, nullptr);
   std::vector<OpRec> rec(nops);
-  b->bytes = 0;
+  bp.bytes = 0;
   for (int i = 0; i < nops; ++i) {
     const ttc_batch_op_t& op = ops[i];
     const int pair = batch_pair(op.type_a, op.type_b);
-    if (pair < 0) {
-      batch_free(b);
-      return TTC_ERROR_UNSUPPORTED;
-    }
-    if (op.dim < 1 || op.dim > TTC_MAX_INPUT_DIM) {
-      batch_free(b);
-      return TTC_ERROR_INVALID_VALUE;
-    }
+    if (pair < 0) return TTC_ERROR_UNSUPPORTED;
+    if (op.dim < 1 || op.dim > TTC_MAX_INPUT_DIM) return TTC_ERROR_INVALID_VALUE;
     bool any_lda = false, any_ldb = false;
     for (int k = 0; k < op.dim; ++k) {
       any_lda |= op.lda[k] != 0;
@@ -450,14 +469,11 @@ extern "C" ttc_status_t ttc_batch_create(ttc_batch_t* out, int nops, const ttc_b
         st = ttc_plan_create(&pl, op.type_a, op.type_b, op.dim, op.perm, op.size, any_lda ? op.lda : nullptr,
                              any_ldb ? op.ldb : nullptr, op.beta_is_zero, &cfg);
       }
-      if (st != TTC_SUCCESS) {
-        batch_free(b);
-        return st;
-      }
+      if (st != TTC_SUCCESS) return st;
       const bool ok_a = pl->va == 1 || pl->va == wide;
       const bool ok_b = pl->vb == 1 || pl->vb == wide;
       if ((ok_a && ok_b) || pass == 1) {
-        b->plans[i] = pl;
+        bp.plans[i] = pl;
         break;
       }
       ttc_plan_destroy(pl);
@@ -465,20 +481,11 @@ extern "C" ttc_status_t ttc_batch_create(ttc_batch_t* out, int nops, const ttc_b
       if (!ok_b) cfg.vec_b = 1;
       for (int k = 0; k < TTC_MAX_DIM; ++k) cfg.tile[k] = 0;  // re-derive the tile for the new widths
     }
-    const ttc_plan_s* pl = b->plans[i];
-    if ((pl->va != 1 && pl->va != wide) || (pl->vb != 1 && pl->vb != wide)) {
-      batch_free(b);
-      return TTC_ERROR_UNSUPPORTED;
-    }
-    if (pl->g.elements > 0 && (!op.A || !op.B)) {
-      batch_free(b);
-      return TTC_ERROR_INVALID_VALUE;
-    }
+    const ttc_plan_s* pl = bp.plans[i];
+    if ((pl->va != 1 && pl->va != wide) || (pl->vb != 1 && pl->vb != wide)) return TTC_ERROR_UNSUPPORTED;
+    if (pl->g.elements > 0 && (!op.A || !op.B)) return TTC_ERROR_INVALID_VALUE;
     const ttc_plan_s* use = pl->g.elements > 0 ? variant_for(pl, op.A, op.B) : pl;
-    if (!use) {
-      batch_free(b);
-      return TTC_ERROR_INVALID_VALUE;
-    }
+    if (!use) return TTC_ERROR_INVALID_VALUE;
     const int wa = use->va > 1 ? 1 : 0, wb = use->vb > 1 ? 1 : 0;
     const int fam = use->family == kFamilyRC ? 1 : 0;
     OpRec& r = rec[i];
@@ -487,34 +494,27 @@ extern "C" ttc_status_t ttc_batch_create(ttc_batch_t* out, int nops, const ttc_b
     r.src = i;
     const int64_t sa = dtype_bytes(op.type_a), sb = dtype_bytes(op.type_b);
     r.weight = r.ntiles ? (double)pl->g.elements * (double)(sa + sb * (pl->beta_is_zero ? 1 : 2)) / r.ntiles : 0.0;
-    b->bytes += pl->g.elements * (sa + sb * (pl->beta_is_zero ? 1 : 2));
+    bp.bytes += pl->g.elements * (sa + sb * (pl->beta_is_zero ? 1 : 2));
   }
   // order the table by kind (stable: ops keep their order inside a kind)
   std::stable_sort(rec.begin(), rec.end(), [](const OpRec& x, const OpRec& y) { return x.kind < y.kind; });
   int dev = 0;
   (void)hipGetDevice(&dev);
   const int cus = device_cus(dev);
-  std::vector<uint32_t> wg_all;
-  std::vector<BatchOpDev> box;
-  std::vector<BatchRCOpDev> rcs;
-  std::vector<TileBase> bases;
-  std::vector<RCTile> rct;
-  std::vector<int32_t> rtab;
-  std::vector<size_t> rec_slot(nops);  // index of each op's first tile record in its family's table
-  std::vector<size_t> tab_slot(nops);  // index of a row x column op's offset tables
-  b->tiles = 0;
+  BatchImage& im = bp.img;
+  bp.tiles = 0;
   for (int j = 0; j < nops;) {
     int e = j;
     uint64_t tiles = 0;
     int lds = 16;
     const bool rc = kind_is_rc(rec[j].kind);
-    const size_t op0 = rc ? rcs.size() : box.size();
+    const size_t op0 = rc ? im.rcs.size() : im.box.size();
     while (e < nops && rec[e].kind == rec[j].kind) {
       OpRec& r = rec[e];
       r.tile_begin = (uint32_t)tiles;
       tiles += r.ntiles;
       const ttc_batch_op_t& op = ops[r.src];
-      const ttc_plan_s* pl = b->plans[r.src];
+      const ttc_plan_s* pl = bp.plans[r.src];
       const ttc_plan_s* use = pl->g.elements > 0 ? variant_for(pl, op.A, op.B) : pl;
       lds = std::max(lds, use->lds_bytes);
       if (rc) {
@@ -527,13 +527,15 @@ extern "C" ttc_status_t ttc_batch_create(ttc_batch_t* out, int nops, const ttc_b
         d.beta = op.beta;
         d.tile_begin = r.tile_begin;
         d.ntiles = r.ntiles;
-        rec_slot[e] = rct.size();  // first tile record of this op
-        rct.resize(rct.size() + r.ntiles);
-        if (r.ntiles) fill_rc_tiles(use->rc, rct.data() + rec_slot[e]);
-        tab_slot[e] = rtab.size();
-        rtab.resize(rtab.size() + 2 * (size_t)use->rc.R + 2 * (size_t)use->rc.C);
-        if (r.ntiles) rc_tables(use->rc, rtab.data() + tab_slot[e]);
-        rcs.push_back(d);
+        const size_t first = im.rct.size();
+        im.rct.resize(first + r.ntiles);
+        if (r.ntiles) fill_rc_tiles(use->rc, im.rct.data() + first);
+        const size_t tab = im.rtab.size();
+        im.rtab.resize(tab + 2 * (size_t)use->rc.R + 2 * (size_t)use->rc.C);
+        if (r.ntiles) rc_tables(use->rc, im.rtab.data() + tab);
+        im.rc_rec.push_back(first);
+        im.rc_tab.push_back(tab);
+        im.rcs.push_back(d);
       } else {
         BatchOpDev d;
         std::memset(&d, 0, sizeof(d));
@@ -545,17 +547,15 @@ extern "C" ttc_status_t ttc_batch_create(ttc_batch_t* out, int nops, const ttc_b
         d.kind = r.kind;
         d.tile_begin = r.tile_begin;
         d.ntiles = r.ntiles;
-        rec_slot[e] = bases.size();
-        bases.resize(bases.size() + r.ntiles);
-        if (r.ntiles) fill_bases(use->p, bases.data() + rec_slot[e]);
-        box.push_back(d);
+        const size_t first = im.bases.size();
+        im.bases.resize(first + r.ntiles);
+        if (r.ntiles) fill_bases(use->p, im.bases.data() + first);
+        im.box_rec.push_back(first);
+        im.box.push_back(d);
       }
       ++e;
     }
-    if (tiles >= (1ull << 31)) {
-      batch_free(b);
-      return TTC_ERROR_UNSUPPORTED;
-    }
+    if (tiles >= (1ull << 31)) return TTC_ERROR_UNSUPPORTED;
     lds = (lds + 15) / 16 * 16;
     if (tiles > 0) {
       BatchGroup grp;
@@ -569,47 +569,68 @@ extern "C" ttc_status_t ttc_batch_create(ttc_batch_t* out, int nops, const ttc_b
       int wg = wg_per_cu > 0 ? wg_per_cu : std::min(occ, 4 * kThreads / threads);
       if (wg < 1) wg = 1;
       grp.nwg = (int)std::min<uint64_t>((uint64_t)cus * wg, tiles);
-      grp.wg0 = (int)wg_all.size();
+      grp.wg0 = (int)im.wg.size();
       std::vector<uint32_t> wgb = split_by_bytes(rec, j, e - j, (uint32_t)tiles, grp.nwg);
-      wg_all.insert(wg_all.end(), wgb.begin(), wgb.end());
-      b->groups.push_back(grp);
-      b->tiles += (int64_t)tiles;
+      im.wg.insert(im.wg.end(), wgb.begin(), wgb.end());
+      bp.groups.push_back(grp);
+      bp.tiles += (int64_t)tiles;
     }
     j = e;
   }
-  b->nwg_total = 0;
-  for (const BatchGroup& g : b->groups) b->nwg_total += g.nwg;
-  if (hipMalloc(&b->d_ops, sizeof(BatchOpDev) * std::max<size_t>(box.size(), 1)) != hipSuccess ||
-      hipMalloc(&b->d_rc_ops, sizeof(BatchRCOpDev) * std::max<size_t>(rcs.size(), 1)) != hipSuccess ||
-      hipMalloc(&b->d_wg, sizeof(uint32_t) * std::max<size_t>(wg_all.size(), 1)) != hipSuccess ||
-      hipMalloc(&b->d_bases, sizeof(TileBase) * std::max<size_t>(bases.size(), 1)) != hipSuccess ||
-      hipMalloc(&b->d_rc_tiles, sizeof(RCTile) * std::max<size_t>(rct.size(), 1)) != hipSuccess ||
-      hipMalloc(&b->d_rc_tables, sizeof(int32_t) * std::max<size_t>(rtab.size(), 1)) != hipSuccess) {
+  bp.nwg_total = 0;
+  for (const BatchGroup& g : bp.groups) bp.nwg_total += g.nwg;
+  im.layout();
+  return TTC_SUCCESS;
+}
+
+// Kernel arguments of group g with the work table at device address base.
+struct GroupArgs {
+  const void* ops;
+  int nops;
+  const uint32_t* wg;
+};
+static GroupArgs group_args(const BatchPlan& bp, const BatchGroup& g, uintptr_t base) {
+  const BatchImage& im = bp.img;
+  GroupArgs a;
+  a.ops = kind_is_rc(g.kind)
+              ? reinterpret_cast<const void*>(base + im.off[BatchImage::kRC] + g.op0 * sizeof(BatchRCOpDev))
+              : reinterpret_cast<const void*>(base + im.off[BatchImage::kBox] + g.op0 * sizeof(BatchOpDev));
+  a.nops = g.nops;
+  a.wg = reinterpret_cast<const uint32_t*>(base + im.off[BatchImage::kWg] + g.wg0 * sizeof(uint32_t));
+  return a;
+}
+
+static ttc_status_t launch_groups(const BatchPlan& bp, uintptr_t base, hipStream_t s) {
+  for (const BatchGroup& g : bp.groups) {
+    GroupArgs a = group_args(bp, g, base);
+    void* args[] = {&a.ops, &a.nops, &a.wg};
+    const hipError_t e = hipLaunchKernel(group_kernel(g.kind), dim3(g.nwg),
+                                         dim3(kind_is_rc(g.kind) ? kRcThreads : kThreads), args,
+                                         (size_t)g.lds_bytes, s);
+    if (e != hipSuccess) return TTC_ERROR_HIP;
+  }
+  return TTC_SUCCESS;
+}
+
+extern "C" ttc_status_t ttc_batch_create(ttc_batch_t* out, int nops, const ttc_batch_op_t* ops, int wg_per_cu) {
+  if (!out || nops < 1 || !ops || wg_per_cu < 0) return TTC_ERROR_INVALID_VALUE;
+  *out = nullptr;
+  ttc_batch_s* b = new (std::nothrow) ttc_batch_s();
+  if (!b) return TTC_ERROR_ALLOC;
+  ttc_status_t st = plan_batch(nops, ops, wg_per_cu, b->bp);
+  if (st != TTC_SUCCESS) {
+    batch_free(b);
+    return st;
+  }
+  const BatchImage& im = b->bp.img;
+  if (hipMalloc(&b->d_image, std::max<size_t>(im.bytes, 256)) != hipSuccess) {
+    b->d_image = nullptr;
     batch_free(b);
     return TTC_ERROR_ALLOC;
   }
-  // point each op record at its tile records
-  {
-    size_t ib = 0, ir = 0;
-    for (int j = 0; j < nops; ++j) {
-      if (kind_is_rc(rec[j].kind)) {
-        rcs[ir].tiles = b->d_rc_tiles + rec_slot[j];
-        rcs[ir].tables = b->d_rc_tables + tab_slot[j];
-        ++ir;
-      } else {
-        box[ib++].bases = b->d_bases + rec_slot[j];
-      }
-    }
-  }
-  auto up = [](void* dst, const void* src, size_t n) {
-    return n == 0 || hipMemcpy(dst, src, n, hipMemcpyHostToDevice) == hipSuccess;
-  };
-  if (!up(b->d_ops, box.data(), sizeof(BatchOpDev) * box.size()) ||
-      !up(b->d_rc_ops, rcs.data(), sizeof(BatchRCOpDev) * rcs.size()) ||
-      !up(b->d_wg, wg_all.data(), sizeof(uint32_t) * wg_all.size()) ||
-      !up(b->d_bases, bases.data(), sizeof(TileBase) * bases.size()) ||
-      !up(b->d_rc_tiles, rct.data(), sizeof(RCTile) * rct.size()) ||
-      !up(b->d_rc_tables, rtab.data(), sizeof(int32_t) * rtab.size())) {
+  std::vector<unsigned char> host;
+  im.render(reinterpret_cast<uintptr_t>(b->d_image), host);
+  if (im.bytes && hipMemcpy(b->d_image, host.data(), im.bytes, hipMemcpyHostToDevice) != hipSuccess) {
     batch_free(b);
     return TTC_ERROR_HIP;
   }
@@ -619,24 +640,15 @@ extern "C" ttc_status_t ttc_batch_create(ttc_batch_t* out, int nops, const ttc_b
 
 extern "C" ttc_status_t ttc_batch_execute(ttc_batch_t b, ttc_stream_t stream) {
   if (!b) return TTC_ERROR_INVALID_VALUE;
-  for (const BatchGroup& g : b->groups) {
-    const bool rc = kind_is_rc(g.kind);
-    const void* ops = rc ? static_cast<const void*>(b->d_rc_ops + g.op0) : static_cast<const void*>(b->d_ops + g.op0);
-    int nops = g.nops;
-    const uint32_t* wg = b->d_wg + g.wg0;
-    void* args[] = {&ops, &nops, &wg};
-    hipError_t e = hipLaunchKernel(group_kernel(g.kind), dim3(g.nwg), dim3(rc ? kRcThreads : kThreads), args,
-                                   (size_t)g.lds_bytes, static_cast<hipStream_t>(stream));
-    if (e != hipSuccess) return TTC_ERROR_HIP;
-  }
-  return TTC_SUCCESS;
+  return launch_groups(b->bp, reinterpret_cast<uintptr_t>(b->d_image), static_cast<hipStream_t>(stream));
 }
 
 extern "C" ttc_status_t ttc_batch_execute_loop(ttc_batch_t b, ttc_stream_t stream) {
   if (!b) return TTC_ERROR_INVALID_VALUE;
-  for (int i = 0; i < b->nops; ++i) {
-    const ttc_batch_op_t& op = b->ops[i];
-    ttc_status_t st = launch_plan(b->plans[i], op.A, op.B, op.alpha, op.beta, static_cast<hipStream_t>(stream));
+  const BatchPlan& bp = b->bp;
+  for (size_t i = 0; i < bp.ops.size(); ++i) {
+    const ttc_batch_op_t& op = bp.ops[i];
+    ttc_status_t st = launch_plan(bp.plans[i], op.A, op.B, op.alpha, op.beta, static_cast<hipStream_t>(stream));
     if (st != TTC_SUCCESS) return st;
   }
   return TTC_SUCCESS;
@@ -661,18 +673,14 @@ extern "C" ttc_status_t ttc_batch_graph_launch(ttc_batch_t b, ttc_stream_t strea
       // the grouped launch: one kernel node per kind, no edges between them
       // (they touch disjoint ops), so the graph may run kinds side by side
       if (hipGraphCreate(&g, 0) != hipSuccess) return TTC_ERROR_HIP;
-      for (const BatchGroup& gr : b->groups) {
-        const bool rc = kind_is_rc(gr.kind);
-        const void* ops = rc ? static_cast<const void*>(b->d_rc_ops + gr.op0)
-                             : static_cast<const void*>(b->d_ops + gr.op0);
-        int nops = gr.nops;
-        const uint32_t* wg = b->d_wg + gr.wg0;
-        void* args[] = {&ops, &nops, &wg};
+      for (const BatchGroup& gr : b->bp.groups) {
+        GroupArgs a = group_args(b->bp, gr, reinterpret_cast<uintptr_t>(b->d_image));
+        void* args[] = {&a.ops, &a.nops, &a.wg};
         hipKernelNodeParams kp;
         std::memset(&kp, 0, sizeof(kp));
         kp.func = const_cast<void*>(group_kernel(gr.kind));
         kp.gridDim = dim3(gr.nwg);
-        kp.blockDim = dim3(rc ? kRcThreads : kThreads);
+        kp.blockDim = dim3(kind_is_rc(gr.kind) ? kRcThreads : kThreads);
         kp.sharedMemBytes = (unsigned)gr.lds_bytes;
         kp.kernelParams = args;
         hipGraphNode_t node;
@@ -695,9 +703,9 @@ extern "C" ttc_status_t ttc_batch_graph_launch(ttc_batch_t b, ttc_stream_t strea
 
 extern "C" ttc_status_t ttc_batch_get_info(ttc_batch_t b, int64_t* tiles, int* workgroups, int64_t* bytes) {
   if (!b) return TTC_ERROR_INVALID_VALUE;
-  if (tiles) *tiles = b->tiles;
-  if (workgroups) *workgroups = b->nwg_total;
-  if (bytes) *bytes = b->bytes;
+  if (tiles) *tiles = b->bp.tiles;
+  if (workgroups) *workgroups = b->bp.nwg_total;
+  if (bytes) *bytes = b->bp.bytes;
   return TTC_SUCCESS;
 }
 

@@ -319,6 +319,35 @@ def _check_tensor(name, t, need, where):
                          % (where, name, have, t.is_contiguous(), need))
 
 
+def _batch_ops_array(ops, where):
+    """ttc_batch_op_t array of op dicts, after checking every tensor against
+    the extent its op spans."""
+    for i, o in enumerate(ops):
+        ea, eb = extent_bytes(o["type_a"], o.get("type_b", o["type_a"]), o["perm"], o["size"], o.get("lda"),
+                              o.get("ldb"))
+        _check_tensor("A", o.get("A"), ea, "%s op %d" % (where, i))
+        _check_tensor("B", o.get("B"), eb, "%s op %d" % (where, i))
+    arr = (BatchOp * len(ops))()
+    for i, o in enumerate(ops):
+        b = arr[i]
+        b.type_a, b.type_b = int(o["type_a"]), int(o.get("type_b", o["type_a"]))
+        b.dim = len(o["perm"])
+        for k in range(b.dim):
+            b.perm[k] = int(o["perm"][k])
+            b.size[k] = int(o["size"][k])
+            if o.get("lda"):
+                b.lda[k] = int(o["lda"][k])
+            if o.get("ldb"):
+                b.ldb[k] = int(o["ldb"][k])
+        b.A, b.B = _ptr(o["A"]), _ptr(o["B"])
+        b.alpha, b.beta = float(o.get("alpha", 1.0)), float(o.get("beta", 0.0))
+        b.beta_is_zero = int(bool(o.get("beta_is_zero", o.get("beta", 0.0) == 0)))
+        cfg = o.get("config")
+        if cfg:
+            b.cfg = cfg if isinstance(cfg, Config) else Config.make(**cfg)
+    return arr
+
+
 class Batch:
     """Grouped launch of many transpositions (ttc_batch_create).
 
@@ -331,30 +360,8 @@ class Batch:
         optional lda, ldb, beta_is_zero, config.  Raises ValueError when a
         tensor is smaller than the extent its op spans."""
         L = _load()
-        for i, o in enumerate(ops):
-            ea, eb = extent_bytes(o["type_a"], o.get("type_b", o["type_a"]), o["perm"], o["size"], o.get("lda"),
-                                  o.get("ldb"))
-            _check_tensor("A", o.get("A"), ea, "Batch op %d" % i)
-            _check_tensor("B", o.get("B"), eb, "Batch op %d" % i)
+        arr = _batch_ops_array(ops, "Batch")
         self._keep = [(o.get("A"), o.get("B")) for o in ops]
-        arr = (BatchOp * len(ops))()
-        for i, o in enumerate(ops):
-            b = arr[i]
-            b.type_a, b.type_b = int(o["type_a"]), int(o.get("type_b", o["type_a"]))
-            b.dim = len(o["perm"])
-            for k in range(b.dim):
-                b.perm[k] = int(o["perm"][k])
-                b.size[k] = int(o["size"][k])
-                if o.get("lda"):
-                    b.lda[k] = int(o["lda"][k])
-                if o.get("ldb"):
-                    b.ldb[k] = int(o["ldb"][k])
-            b.A, b.B = _ptr(o["A"]), _ptr(o["B"])
-            b.alpha, b.beta = float(o.get("alpha", 1.0)), float(o.get("beta", 0.0))
-            b.beta_is_zero = int(bool(o.get("beta_is_zero", o.get("beta", 0.0) == 0)))
-            cfg = o.get("config")
-            if cfg:
-                b.cfg = cfg if isinstance(cfg, Config) else Config.make(**cfg)
         self._h = ctypes.c_void_p()
         _check(L.ttc_batch_create(ctypes.byref(self._h), len(ops), arr, int(wg_per_cu)), "ttc_batch_create")
         self._lib = L

@@ -75,7 +75,7 @@ def printHelp():
     print("   --dataType=[s,d,c,z,sd,ds,cz,zc,b,h,sb,sh,bs,hs,sq,sr,bq,br,hq,hr or typeA->typeB]".ljust(60),
           "element types ('xy': A has type x, B type y; b = bfloat16, h = half, q/r = fp8 e4m3/e5m2)")
     print("   --architecture=hip".ljust(60), "target (hip = gfx950; 'cuda' is accepted as an alias)")
-    print("   --streamingStores".ljust(60), "non-temporal stores when beta == 0 (the default on the GPU)")
+    print("   --streamingStores".ljust(60), "try non-temporal B stores in the search (beta == 0 only)")
     print("   --threadsPerBlock=256".ljust(60), "work-group size (256 is the only value)")
     print("   --blockings=<a>x<b>[,...]".ljust(60), "restrict the search to tiles of these A x B run lengths")
     print("   --generateOnly".ljust(60), "emit the heuristic choice, no timing")
@@ -280,6 +280,15 @@ def generateTransposition(ttcArgs):
     lda = ttcArgs.lda if ttcArgs.lda else None
     ldb = ttcArgs.ldb if ttcArgs.ldb else None
     bz = ttcArgs.beta == 0
+    # --streamingStores: non-temporal B stores join the search (beta == 0
+    # only, ttc/ttc_util.py:301-314); --blockings: the search only takes
+    # tiles of these A x B run lengths (ttc/ttc.py:1296-1297, 827, 849)
+    streaming = ttc_util.streamingStoresApplicable(ttcArgs.ldb, ttcArgs.size, ttcArgs.idxPerm, ttcArgs.beta, 1,
+                                                   ttc_util.FLOAT_TYPES[ttcArgs.floatTypeB][1],
+                                                   int(bool(ttcArgs.streamingStores)))
+    blockings = [(int(a), int(b)) for a, b in (ttcArgs.blockings or [])]
+    if any(a < 1 or b < 1 for a, b in blockings):
+        _fail("--blockings: run lengths must be positive")
     try:
         base_plan = rt.Plan(ta, tb, ttcArgs.idxPerm, ttcArgs.size, lda, ldb, bz)
     except rt.TTCError as e:
@@ -343,7 +352,9 @@ def generateTransposition(ttcArgs):
                 sys.stdout.flush()
         cands_limit = ttcArgs.maxNumImplementations
         results = autotune.tune(ta, tb, ttcArgs.idxPerm, ttcArgs.size, lda, ldb, ttcArgs.alpha, ttcArgs.beta,
-                                cands_limit, progress=progress, hot_a=bool(ttcArgs.hotA), hot_b=bool(ttcArgs.hotB))
+                                cands_limit, progress=progress, nt_stores=bool(streaming),
+                                hot_a=bool(ttcArgs.hotA), hot_b=bool(ttcArgs.hotB),
+                                blockings=blockings or None)
         measuringTime = _time.time() - t0
         if not results:
             _fail("no variant could be measured")
@@ -354,9 +365,15 @@ def generateTransposition(ttcArgs):
                 results[0]["info"]["tile"], results[0]["info"]["vec_a"], results[0]["info"]["vec_b"],
                 results[0]["GiBs"], results[0]["GBs"]))
     if best_cfg is None:  # --generateOnly without a stored solution: the planner's choice
-        info = base_plan.info()
+        cfg0 = {"run_a": blockings[0][0], "run_b": blockings[0][1]} if blockings else {}
+        if streaming:
+            cfg0["nt_store"] = 1
+        try:
+            info = rt.Plan(ta, tb, ttcArgs.idxPerm, ttcArgs.size, lda, ldb, bz, cfg0 or None).info()
+        except rt.TTCError:
+            info = base_plan.info()
         best_cfg = {"tile": info["tile"], "vec_a": info["vec_a"], "vec_b": info["vec_b"], "family": info["family"],
-                    "tile_order": info["tile_order"]}
+                    "tile_order": info["tile_order"], "nt_store": 1 if info["nt_store"] else 0}
 
     # ----------------------------------------------------- emit artefact
     directory = os.path.join(os.getcwd(), "ttc_transpositions")

@@ -7,7 +7,6 @@ same.  Floating-point type strings follow the reference ("float", "double",
 ("bfloat16", "half", "fp8_e4m3", "fp8_e5m2").
 """
 import copy
-import os
 
 OKGREEN = '\033[92m'
 FAIL = '\033[91m'
@@ -29,24 +28,7 @@ FLOAT_TYPES = {
 
 def getVersion():
     """ttc/ttc_util.py:10-11 (ttc_amd's own version)."""
-    return [0, 1, 0]
-
-
-def createTmpDirectory(base="tmp"):
-    """ttc/ttc_util.py:13-20."""
-    i = 0
-    while os.path.exists(base + "%d" % i):
-        i += 1
-    d = base + "%d" % i
-    os.makedirs(d)
-    return d + "/"
-
-
-def getFloatTypeSize(floatType):
-    """Bytes per element (ttc/ttc_util.py:23-34)."""
-    if floatType not in FLOAT_TYPES:
-        raise ValueError("[TTC] ERROR: unknown data type %r" % (floatType,))
-    return FLOAT_TYPES[floatType][1]
+    return [0, 2, 0]
 
 
 def dtypeCode(floatType):
@@ -166,31 +148,40 @@ def getArchitecture(arch):
     return "host"
 
 
-def getCostLoop(loopPerm, perm, size):
-    """Loop-order cost model, restated from ttc/ttc_util.py:167-209:
-    sum over loops of totalSize*(extent-1)*((lastOffsetA-lda0)^2 +
-    (lastOffsetB-ldb0)^2) with dense strides."""
-    lda = [1]
-    ldb = [1]
-    for i in range(1, len(size)):
-        lda.append(lda[i - 1] * size[i - 1])
-        ldb.append(ldb[i - 1] * size[perm[i - 1]])
-    totalSize = 1
-    cost = 0
-    for i in range(len(loopPerm)):
-        loopIdx = loopPerm[i]
-        idxB = perm.index(loopIdx)
-        lda0 = lda[loopIdx]
-        ldb0 = ldb[idxB]
-        lastOffsetA = 0
-        lastOffsetB = 0
-        for l in range(i + 1, len(loopPerm)):
-            nxt = loopPerm[l]
-            lastOffsetA += lda[nxt] * (size[nxt] - 1)
-            lastOffsetB += ldb[perm.index(nxt)] * (size[nxt] - 1)
-        cost += totalSize * (size[loopIdx] - 1) * ((lastOffsetA - lda0) ** 2. + (lastOffsetB - ldb0) ** 2.)
-        totalSize *= size[loopIdx]
+def walkCost(loops):
+    """TTC's loop-order cost (ttc/ttc_util.py:167-209) of a loop nest given
+    as (extent, A stride, B stride) per loop, outermost first.  Whenever
+    loop i advances, every loop inside it wraps from its last iteration
+    back to 0, so the address jumps by  stride_i - sum_inner stride_l*(n_l-1)
+    in A and likewise in B; such advances happen  (n_i - 1) * prod_outer n
+    times.  The cost is the sum of the squared A and B jumps over all of
+    them: orders whose inner loops stay near the previous address score low."""
+    cost = 0.0
+    count = 1
+    for i, (n, sa, sb) in enumerate(loops):
+        wrap_a = sum(sa_l * (n_l - 1) for n_l, sa_l, _ in loops[i + 1:])
+        wrap_b = sum(sb_l * (n_l - 1) for n_l, _, sb_l in loops[i + 1:])
+        cost += count * (n - 1) * (float(sa - wrap_a) ** 2 + float(sb - wrap_b) ** 2)
+        count *= n
     return cost
+
+
+def getCostLoop(loopPerm, perm, size, lda=None, ldb=None):
+    """Cost of visiting the elements of the transposition (perm, size) with
+    the loops in loopPerm (A index labels, outermost first): walkCost over
+    their element strides in A and in B (dense, or from lda / ldb as
+    ttc/transpose.py:443-473 defines them)."""
+    d = len(size)
+    LA = list(lda) if lda else list(size)
+    LB = list(ldb) if ldb else [size[perm[q]] for q in range(d)]
+    sA = [1] * d
+    for k in range(1, d):
+        sA[k] = sA[k - 1] * LA[k - 1]
+    sBpos = [1] * d
+    for q in range(1, d):
+        sBpos[q] = sBpos[q - 1] * LB[q - 1]
+    sB = [sBpos[perm.index(k)] for k in range(d)]  # B stride of A index k
+    return walkCost([(size[k], sA[k], sB[k]) for k in loopPerm])
 
 
 def fuseIndices(size, perm, loopPermutations, lda, ldb):
@@ -259,15 +250,12 @@ def getCompilerVersion(compiler):
 
 def streamingStoresApplicable(ldb, size, perm, beta, alignmentRequirementBytes, floatSizeBytes,
                               useStreamingStores):
-    """ttc/ttc_util.py:301-314 (CPU rule; on the GPU, beta == 0 selects
-    non-temporal stores unless disabled)."""
-    leadingDimension = size[perm[0]]
-    if len(ldb) > 0:
-        leadingDimension = ldb[0]
-    if useStreamingStores == 1 and beta == 0:
-        if perm[0] == 0:
-            return 1
-        if (leadingDimension * floatSizeBytes) % alignmentRequirementBytes == 0:
-            return 1
+    """1 when B's stores may be non-temporal (ttc/ttc_util.py:301-314): they
+    must be asked for (--streamingStores) and beta must be 0, so that B is
+    write-only; B's leading dimension must also be a multiple of the store
+    alignment unless dimension 0 stays in place.  The GPU's non-temporal
+    buffer stores take any alignment (alignmentRequirementBytes = 1)."""
+    if not useStreamingStores or beta != 0:
         return 0
-    return 0
+    lead = ldb[0] if len(ldb) > 0 else size[perm[0]]
+    return 1 if perm[0] == 0 or (lead * floatSizeBytes) % alignmentRequirementBytes == 0 else 0
