@@ -55,7 +55,8 @@ typedef enum {
   TTC_ERROR_UNSUPPORTED = 2,   /* type pair or shape the kernels do not cover */
   TTC_ERROR_HIP = 3,           /* a HIP runtime call failed */
   TTC_ERROR_ALLOC = 4,         /* host or device allocation failed */
-  TTC_ERROR_NO_DEVICE = 5      /* no usable gfx950 device */
+  TTC_ERROR_NO_DEVICE = 5,     /* no usable gfx950 device */
+  TTC_ERROR_WORKSPACE = 6      /* the work table exceeds TTC_BATCH_WORKSPACE_BYTES (ttc_batch_execute_noalloc) */
 } ttc_status_t;
 
 typedef struct ttc_plan_s* ttc_plan_t;
@@ -204,6 +205,23 @@ ttc_status_t ttc_batch_execute_loop(ttc_batch_t batch, ttc_stream_t stream);
 /* Number of tiles / work-groups / algorithmic bytes of the grouped launch. */
 ttc_status_t ttc_batch_get_info(ttc_batch_t batch, int64_t* tiles, int* workgroups, int64_t* bytes);
 ttc_status_t ttc_batch_destroy(ttc_batch_t batch);
+
+/* One grouped launch of `ops` that allocates no device memory.  No
+ * counterpart in the reference: it is for callers that cannot call
+ * hipMalloc (code under stream capture, tight serving loops, harnesses that
+ * forbid allocation).  It does what ttc_batch_create + ttc_batch_execute +
+ * ttc_batch_destroy do, with the same kernels and results, except that the
+ * work table goes into a fixed device workspace the library owns, one per
+ * GPU, of TTC_BATCH_WORKSPACE_BYTES, held until the process ends.  The table
+ * is built on the host (host heap only) and copied with hipMemcpyAsync on
+ * `stream`, ordered after that stream's earlier work; the call returns once
+ * the kernels are enqueued.  A table larger than the workspace (about 24
+ * bytes per tile plus about 1 KiB per op) returns TTC_ERROR_WORKSPACE.
+ * Every call on one GPU shares that GPU's workspace: calls on one stream
+ * are ordered, but calls that overlap on one GPU (different streams or host
+ * threads, without ordering between them) race. */
+#define TTC_BATCH_WORKSPACE_BYTES (16 << 20)
+ttc_status_t ttc_batch_execute_noalloc(int nops, const ttc_batch_op_t* ops, int wg_per_cu, ttc_stream_t stream);
 
 #ifdef __cplusplus
 }

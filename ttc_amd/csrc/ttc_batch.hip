@@ -19,6 +19,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <type_traits>
 #include <vector>
@@ -713,4 +714,51 @@ extern "C" ttc_status_t ttc_batch_destroy(ttc_batch_t b) {
   if (!b) return TTC_ERROR_INVALID_VALUE;
   batch_free(b);
   return TTC_SUCCESS;
+}
+
+// ------------------------------------------------------- no device allocation
+// The fixed workspace of ttc_batch_execute_noalloc: every GPU the library
+// runs on holds its own copy until the process ends.  In host code the name
+// `scratch` is only a stand-in, never a device address: scratch_address()
+// looks up the current device's copy once (hipGetSymbolAddress) and all
+// later uses (the table copy, the kernels' arguments) go through that.
+static __device__ __align__(256) unsigned char scratch[TTC_BATCH_WORKSPACE_BYTES];
+
+static constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices];
+
+static ttc_status_t scratch_address(void** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess) return TTC_ERROR_NO_DEVICE;
+  if (dev < 0 || dev >= kMaxDevices) return TTC_ERROR_NO_DEVICE;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return TTC_ERROR_HIP;
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return TTC_SUCCESS;
+}
+
+extern "C" ttc_status_t ttc_batch_execute_noalloc(int nops, const ttc_batch_op_t* ops, int wg_per_cu,
+                                                  ttc_stream_t stream) {
+  if (nops < 1 || !ops || wg_per_cu < 0) return TTC_ERROR_INVALID_VALUE;
+  BatchPlan bp;
+  ttc_status_t st = plan_batch(nops, ops, wg_per_cu, bp);
+  if (st != TTC_SUCCESS) return st;
+  if (bp.img.bytes > (size_t)TTC_BATCH_WORKSPACE_BYTES) return TTC_ERROR_WORKSPACE;
+  if (bp.groups.empty()) return TTC_SUCCESS;  // every op is empty
+  void* base = nullptr;
+  st = scratch_address(&base);
+  if (st != TTC_SUCCESS) return st;
+  std::vector<unsigned char> host;
+  bp.img.render(reinterpret_cast<uintptr_t>(base), host);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  // Ordered on the stream after the previous call's kernels; the pageable
+  // host table is consumed before the copy call returns.
+  if (hipMemcpyAsync(base, host.data(), bp.img.bytes, hipMemcpyHostToDevice, s) != hipSuccess)
+    return TTC_ERROR_HIP;
+  return launch_groups(bp, reinterpret_cast<uintptr_t>(base), s);
 }

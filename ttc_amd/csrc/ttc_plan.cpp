@@ -760,6 +760,7 @@ extern "C" const char* ttc_status_string(ttc_status_t s) {
     case TTC_ERROR_HIP: return "HIP runtime error";
     case TTC_ERROR_ALLOC: return "allocation failed";
     case TTC_ERROR_NO_DEVICE: return "no usable gfx950 device";
+    case TTC_ERROR_WORKSPACE: return "work table larger than the fixed device workspace (TTC_BATCH_WORKSPACE_BYTES)";
   }
   return "unknown status";
 }

@@ -21,7 +21,9 @@ DTYPE_NAMES = {S: "s", D: "d", C: "c", Z: "z", BF16: "bf16", F16: "f16", E4M3: "
 DTYPE_BYTES = {S: 4, D: 8, C: 8, Z: 16, BF16: 2, F16: 2, E4M3: 1, E5M2: 1}
 
 # ttc_status_t
-SUCCESS, ERROR_INVALID_VALUE, ERROR_UNSUPPORTED, ERROR_HIP, ERROR_ALLOC, ERROR_NO_DEVICE = range(6)
+SUCCESS, ERROR_INVALID_VALUE, ERROR_UNSUPPORTED, ERROR_HIP, ERROR_ALLOC, ERROR_NO_DEVICE, ERROR_WORKSPACE = range(7)
+
+BATCH_WORKSPACE_BYTES = 16 << 20  # TTC_BATCH_WORKSPACE_BYTES
 
 SCALE_DELAYED, SCALE_CURRENT = 0, 1
 
@@ -97,7 +99,7 @@ EXPORTS = [
     "ttc_version", "ttc_status_string", "ttc_dtype_size", "ttc_fuse_indices", "ttc_plan_create",
     "ttc_plan_get_info", "ttc_plan_destroy", "ttc_execute", "ttc_execute_fp8", "ttc_legacy_transpose",
     "ttc_batch_create", "ttc_batch_execute", "ttc_batch_graph_launch", "ttc_batch_execute_loop",
-    "ttc_batch_get_info", "ttc_batch_destroy",
+    "ttc_batch_get_info", "ttc_batch_destroy", "ttc_batch_execute_noalloc",
 ]
 
 _lib = None
@@ -150,6 +152,8 @@ def _load():
     L.ttc_batch_get_info.restype = ctypes.c_int
     L.ttc_batch_destroy.argtypes = [vp]
     L.ttc_batch_destroy.restype = ctypes.c_int
+    L.ttc_batch_execute_noalloc.argtypes = [ctypes.c_int, ctypes.POINTER(BatchOp), ctypes.c_int, vp]
+    L.ttc_batch_execute_noalloc.restype = ctypes.c_int
     _lib = L
     return L
 
@@ -346,6 +350,16 @@ def _batch_ops_array(ops, where):
         if cfg:
             b.cfg = cfg if isinstance(cfg, Config) else Config.make(**cfg)
     return arr
+
+
+def batch_execute_noalloc(ops, stream=None, wg_per_cu=0):
+    """One grouped launch of `ops` (dicts as for Batch) through
+    ttc_batch_execute_noalloc: no device allocation, the work table goes to
+    the library's fixed per-GPU workspace (BATCH_WORKSPACE_BYTES).  Calls on
+    one GPU that overlap without stream ordering race for that workspace."""
+    L = _load()
+    arr = _batch_ops_array(ops, "batch_execute_noalloc")
+    _check(L.ttc_batch_execute_noalloc(len(ops), arr, int(wg_per_cu), _stream(stream)), "ttc_batch_execute_noalloc")
 
 
 class Batch:
