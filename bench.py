@@ -281,17 +281,22 @@ def cpu_model():
     return "unknown CPU"
 
 
-def cpu_oracle_rate(oracle, np, ops, nthreads, trash, reps=4):
-    """Bytes and seconds of the oracle over (tA, A, B, perm, size, beta)
-    ops: min of `reps` runs of each, each after a cache-trash sweep
-    (ttc/transposeGenerator.py:1493-1502, min of nRepeat as 452-837)."""
+def cpu_oracle_rate(oracle, np, ops, nthreads, trash, reps=4, blocked=False):
+    """Bytes and seconds over (tA, A, B, perm, size, beta) ops: min of
+    `reps` runs of each, each after a cache-trash sweep
+    (ttc/transposeGenerator.py:1493-1502, min of nRepeat as 452-837).
+    blocked: the blocked, vectorised baseline instead of the oracle's
+    scalar loop nest."""
     tot_bytes, tot_s = 0, 0.0
     for t, A, B, perm, size, beta in ops:
         best = None
         for _ in range(reps):
             oracle.trash_cache(trash, nthreads)
             t0 = time.perf_counter()
-            oracle.transpose(t, t, A, B, 1.0, beta, perm, size, nthreads=nthreads)
+            if blocked:
+                oracle.blocked_transpose(t, A, B, 1.0, beta, perm, size, nthreads=nthreads)
+            else:
+                oracle.transpose(t, t, A, B, 1.0, beta, perm, size, nthreads=nthreads)
             dt = time.perf_counter() - t0
             best = dt if best is None else min(best, dt)
         n = int(np.prod(size))
@@ -301,18 +306,20 @@ def cpu_oracle_rate(oracle, np, ops, nthreads, trash, reps=4):
 
 
 def cpu_baseline(budget_s):
-    """The oracle (C++ restatement of TTC's reference loop nest, OpenMP on
-    the host threads this process may use) on a bounded sample of the same
-    workload: C2 cases in fp32 and fp64, beta 0 and 1, min of 4 reps each
-    after a cache-trash sweep of twice the host's L3 (at least 512 MiB),
-    until `budget_s` seconds of CPU work are spent."""
+    """The CPU path on the host threads this process may use, on a bounded
+    sample of the same workload (C2 cases in fp32 and fp64, beta 0 and 1,
+    min of 4 reps each after a cache-trash sweep of twice the host's L3, at
+    least 512 MiB, until `budget_s` seconds are spent): TTC's blocked,
+    vectorised CPU method (oracle/ttc_cpu_blocked.cpp, AVX-512 or AVX2
+    micro-kernels; `value`) and, beside it, the oracle's scalar loop nest
+    (the reference code the generator emits for checking)."""
     import numpy as np
 
     from oracle import oracle
     from ttc_amd import workloads as wl
     nthreads = host_threads()
     cases = wl.plain_c2()
-    done, tot_bytes, tot_s = [], 0, 0.0
+    done, tot = [], {"blocked": [0, 0.0], "oracle": [0, 0.0]}
     trash_bytes = max(512 << 20, 2 * l3_bytes())
     trash = np.ones(trash_bytes // 8)
     t_start = time.perf_counter()
@@ -322,9 +329,7 @@ def cpu_baseline(budget_s):
             break
         case = cases[k]
         size, perm = case["sizes"][es], case["perm"]
-        lda = case["lda"][es] if case["lda"] else None
-        ldb = case["ldb"][es] if case["ldb"] else None
-        if lda or ldb:
+        if case["lda"] or case["ldb"]:
             continue  # the padded case: dense extents only here
         n = int(np.prod(size))
         rng = np.random.default_rng(k)
@@ -332,16 +337,21 @@ def cpu_baseline(budget_s):
         A = rng.random(n, dtype=dt)
         B = rng.random(n, dtype=dt)
         t = oracle.S if es == 4 else oracle.D
-        b, s_ = cpu_oracle_rate(oracle, np, [(t, A, B, perm, size, 0.0), (t, A, B, perm, size, 1.0)], nthreads,
-                                trash)
-        tot_bytes += b
-        tot_s += s_
+        ops = [(t, A, B, perm, size, 0.0), (t, A, B, perm, size, 1.0)]
+        for name in ("blocked", "oracle"):
+            b, s_ = cpu_oracle_rate(oracle, np, ops, nthreads, trash, blocked=name == "blocked")
+            tot[name][0] += b
+            tot[name][1] += s_
         done.append("k%d/%s" % (k, "f32" if es == 4 else "f64"))
         del A, B
-    return {"value": round(tot_bytes / tot_s / 1e9, 3), "unit": "GB/s", "cores": nthreads, "kind": "port",
-            "sample": "oracle loop nest (OpenMP, %d threads, %s) on C2 cases %s, beta 0 and 1 each, min of 4 reps "
-                      "after a %d MiB cache-trash sweep (2x L3)" % (nthreads, cpu_model(), ",".join(done),
-                                                                  trash_bytes >> 20)}
+    isa = {2: "AVX-512", 1: "AVX2"}.get(oracle.blocked_isa(), "none")
+    rate = {k: v[0] / v[1] / 1e9 for k, v in tot.items()}
+    return {"value": round(rate["blocked"], 3), "unit": "GB/s", "cores": nthreads, "kind": "port",
+            "oracle_loop_nest_GBps": round(rate["oracle"], 3),
+            "sample": "TTC's blocked CPU method (%s micro-kernels, OpenMP %d threads, %s) on C2 cases %s, beta 0 "
+                      "and 1 each, min of 4 reps after a %d MiB cache-trash sweep (2x L3); the oracle's scalar loop "
+                      "nest on the same ops: oracle_loop_nest_GBps" % (isa, nthreads, cpu_model(), ",".join(done),
+                                                                       trash_bytes >> 20)}
 
 
 def run_c1(torch, rt, wl, dev, stream, reps=20, nbuf=4):

@@ -30,11 +30,55 @@ _lib = None
 
 
 def build(force=False):
-    """Compile the oracle with its Makefile (g++, OpenMP)."""
-    if force or not os.path.exists(_LIB) or (
-            os.path.getmtime(_LIB) < os.path.getmtime(os.path.join(_HERE, "ttc_oracle.cpp"))):
+    """Compile the oracle and the blocked CPU baseline with their Makefile
+    (g++, OpenMP)."""
+    stale = any(not os.path.exists(lib) or os.path.getmtime(lib) < os.path.getmtime(os.path.join(_HERE, src))
+                for lib, src in ((_LIB, "ttc_oracle.cpp"), (_BLOCKED_LIB, "ttc_cpu_blocked.cpp")))
+    if force or stale:
         subprocess.check_call(["make", "-s", "-C", _HERE, "-B" if force else "all"])
     return _LIB
+
+
+# ------------------------------------------------ blocked CPU baseline
+# ttc_cpu_blocked.cpp: TTC's blocked, vectorised CPU method (W x W in-register
+# micro-kernels, AVX2 / AVX-512), a stronger CPU baseline than the scalar
+# loop nest; bit-identical to the oracle for s and d.
+_BLOCKED_LIB = os.path.join(_HERE, "libttc_cpu_blocked.so")
+_blocked = None
+
+
+def _blocked_lib():
+    global _blocked
+    if _blocked is None:
+        if not os.path.exists(_BLOCKED_LIB):
+            build()
+        L = ctypes.CDLL(_BLOCKED_LIB)
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.ttc_cpu_blocked_transpose.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
+                                                ctypes.c_double, ctypes.c_int, ctypes.c_int, ip, ip, ip, ip,
+                                                ctypes.c_int, ctypes.c_int]
+        L.ttc_cpu_blocked_transpose.restype = ctypes.c_int
+        L.ttc_cpu_blocked_isa.restype = ctypes.c_int
+        _blocked = L
+    return _blocked
+
+
+def blocked_isa():
+    """2: AVX-512F micro-kernels, 1: AVX2, 0: neither on this host."""
+    return _blocked_lib().ttc_cpu_blocked_isa()
+
+
+def blocked_transpose(t, A, B, alpha, beta, perm, size, lda=None, ldb=None, beta_is_zero=None, nthreads=0, isa=0):
+    """The blocked CPU baseline, in place on B (t: S or D, A and B of that type)."""
+    if beta_is_zero is None:
+        beta_is_zero = (beta == 0)
+    assert A.flags.c_contiguous and B.flags.c_contiguous
+    r = _blocked_lib().ttc_cpu_blocked_transpose(t, A.ctypes.data, B.ctypes.data, float(alpha), float(beta),
+                                                 int(bool(beta_is_zero)), len(perm), _iarr(perm), _iarr(size),
+                                                 _iarr(lda), _iarr(ldb), int(nthreads), int(isa))
+    if r != 0:
+        raise ValueError("blocked baseline rejected the arguments (code %d)" % r)
+    return B
 
 
 def lib():
