@@ -214,6 +214,11 @@ def test_edge_shapes(gpu):
     check(gpu, S, S, [2, 1, 0], [1, 1, 5], 1.0, 0.0)
     check(gpu, Z, Z, [3, 2, 1, 0], [2, 1, 3, 1], 1.0, 1.0)
     check(gpu, S, S, [0, 1, 2, 3, 4, 5], [2, 3, 4, 5, 6, 7], 1.0, 0.0)  # identity -> fused rank 1
+    # a stride-1 dimension of extent 1 with a padded leading dimension: the
+    # remaining dimension 0 is in place but strided (B's padding must stay)
+    check(gpu, BF16, BF16, [1, 0, 2], [4, 1, 16], 0.6, 1.0, [5, 1, 18], [3, 6, 18])
+    check(gpu, S, S, [1, 0, 2], [4, 1, 16], 1.5, 0.0, [5, 1, 18], [3, 6, 18])
+    check(gpu, D, D, [1, 0, 2], [1, 4, 16], 1.5, 0.5, [3, 4, 16], None)
     # rank 16 before fusion, several size-1 dimensions
     perm = [15 - i for i in range(16)]
     size = [2, 1, 2, 1, 2, 1, 2, 1, 1, 2, 1, 1, 1, 2, 1, 1]

@@ -267,6 +267,19 @@ def test_row_copy_family_for_perm0():
     assert rt.Plan(S, S, [0, 2, 1], [7, 5, 3], beta_is_zero=False).info()["vec_a"] == 1
     assert rt.Plan(S, S, [0, 1, 2], [7, 5, 3]).info()["family"] == 3  # identity: one row
     assert rt.Plan(S, S, [0, 2, 1], [8, 5, 7], config={"family": 1}).info()["family"] == 1
+    # B's stride-1 dimension has extent 1 and a padded leading dimension:
+    # once it is dropped, dimension 0 is in place but strided on B (3), so
+    # rows are not contiguous and the row copy must not take it
+    for ta in (S, rt.BF16):
+        i = rt.Plan(ta, ta, [1, 0, 2], [4, 1, 16], [5, 1, 18], [3, 6, 18]).info()
+        assert i["perm"] == [0, 1] and i["stride_b"] == [3, 18], i
+        assert i["family"] == 1 and i["vec_a"] == i["vec_b"] == 1, i
+        with pytest.raises(rt.TTCError):
+            rt.Plan(ta, ta, [1, 0, 2], [4, 1, 16], [5, 1, 18], [3, 6, 18], config={"family": 3})
+    # the same on A's side: A's stride-1 dimension has extent 1
+    i = rt.Plan(S, S, [1, 0, 2], [1, 4, 16], [3, 4, 16], None).info()
+    assert i["perm"] == [0, 1] and i["stride_a"] == [3, 12] and i["stride_b"] == [1, 4], i
+    assert i["family"] == 1 and i["vec_a"] == 1, i
     with pytest.raises(rt.TTCError):
         rt.Plan(S, S, [1, 0, 2], [8, 5, 7], config={"family": 3})
     # explicit widths of the row copy: one width, from its own list

@@ -103,7 +103,10 @@ def candidates(type_a, type_b, perm, size, lda=None, ldb=None, beta_is_zero=Fals
                 vecs.append((va, vb))
     tiles = {}
     runs = [1 << k for k in range(0, 14)]
-    targets = list(blockings) if blockings else [(ra, rb) for ra in runs for rb in runs if ra * rb <= 4 * 16384]
+    # (0, 0): the planner's own search for each family and widths, which
+    # also weighs tiles covering short leading extents whole
+    targets = list(blockings) if blockings else [(0, 0)] + [(ra, rb) for ra in runs for rb in runs
+                                                           if ra * rb <= 4 * 16384]
     for fam in (2, 1):
         for va, vb in vecs:
             for ra, rb in targets:

@@ -5,15 +5,18 @@
 //
 // Work table: one record per op (its tile parameters, pointers, scalars) and
 // one record per tile (origin offsets, partial-tile limits), ops grouped by
-// kernel kind (kernel family, type pair, vector widths, beta read or not),
-// each group's tiles numbered consecutively, and a split of each group's tile
-// range into one contiguous slice per work-group, balanced by bytes moved.
-// One persistent launch per kind present walks that kind's slices.  Ops whose
-// tiles have the row x column shape run the row x column kernel (a
-// work-group rebuilds its LDS offset tables when its slice crosses into the
-// next op); the others run box tiles.  (One kernel for all kinds would carry
-// the register needs of its most demanding kind into every op: measured on
-// gfx950 it either ran at one wave per SIMD or spilled.)
+// kernel family and type pair and ordered by variant (vector widths, beta
+// read or not) inside a group, each group's tiles numbered consecutively,
+// and a split of each group's tile range into one contiguous slice per
+// work-group, balanced by bytes moved.  One persistent launch per group
+// walks that group's slices, each op through its variant's walk (the
+// kernels are in ttc_batch_kernels.hpp).  Each op takes the family its own
+// plan chose: row x column tiles (a work-group rebuilds its LDS offset
+// tables when its slice crosses into the next op), row copies (dimension 0
+// in place; the unit of work is a chunk of vectors) or box tiles.  One
+// launch per group rather than per (group, variant): a graph of many small
+// independent launches runs them a few at a time (the hardware queues),
+// each with its own tail.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -21,26 +24,14 @@
 #include <cstring>
 #include <mutex>
 #include <new>
-#include <type_traits>
 #include <vector>
 
+#include "ttc_batch_kernels.hpp"
 #include "ttc_launch.hpp"
 #include "ttc_plan.h"
 #include "ttc_runtime.h"
 
 namespace ttc {
-
-struct BatchOpDev {
-  TileParams p;
-  const void* A;
-  void* B;
-  double alpha, beta;
-  int32_t kind;
-  int32_t pad_;
-  uint32_t tile_begin;  // first tile of this op in its group
-  uint32_t ntiles;
-  const TileBase* bases;  // this op's tile records (ntiles of them)
-};
 
 // Host version of the tile-origin decomposition (tiles numbered in A order
 // over the tile grid), filling one TileBase per tile.
@@ -62,180 +53,24 @@ static void fill_bases(const TileParams& p, TileBase* out) {
   }
 }
 
-// Kinds handled by the grouped kernels: same-type pairs, vector width 1 or
-// the widest on each side, beta read or not, box or row x column tiles.
-// kind = (((family * kBPairs + pair) * 2 + wide_a) * 2 + wide_b) * 2 + beta_is_zero
-enum BatchPair : int { kBS = 0, kBD = 1, kBBF16 = 2, kBF16 = 3, kBC = 4, kBZ = 5, kBPairs = 6 };
-constexpr int kKindsPerFamily = kBPairs * 8;
+// Kernel of group key = family * kBPairs + pair (family 0 box tiles, 1 row
+// x column tiles, 2 row copies).
+enum GroupFamily { kGroupBox = 0, kGroupRC = 1, kGroupRow = 2 };
+static int group_of(int kind) { return kind / kVariantStride; }
+static int group_family(int group) { return group / kBPairs; }
 
-// Op record of a row x column group.
-struct BatchRCOpDev {
-  RCParams p;
-  const void* A;
-  void* B;
-  double alpha, beta;
-  uint32_t tile_begin;    // first tile of this op in its group
-  uint32_t ntiles;
-  const RCTile* tiles;    // this op's tile records
-  const int32_t* tables;  // its offset tables: rowA[R], rowB[R], colA[C], colB[C]
-};
-using KRCOp = const __attribute__((address_space(4))) BatchRCOpDev;
-
-__device__ __forceinline__ int32_t uni(int32_t x) { return __builtin_amdgcn_readfirstlane(x); }
-__device__ __forceinline__ uint32_t uni(uint32_t x) { return (uint32_t)__builtin_amdgcn_readfirstlane((int32_t)x); }
-
-// Parameter records are copied out of the work table whole, one 32-bit word
-// at a time, so that no field can be left behind when a field is added (a
-// field-by-field copy once missed one, and a kernel then read an
-// uninitialised value).  The compiler drops the words a kernel never reads.
-static_assert(sizeof(TileParams) % 4 == 0 && std::is_trivially_copyable<TileParams>::value,
-              "TileParams is copied as 32-bit words");
-static_assert(sizeof(RCParams) % 4 == 0 && std::is_trivially_copyable<RCParams>::value,
-              "RCParams is copied as 32-bit words");
-
-// A box op's parameters, each word marked wave-uniform (readfirstlane), so
-// that they live in scalar registers as kernel arguments do in the
-// single-op kernel.
-__device__ __forceinline__ void uniform_params(const TileParams& src, TileParams& p) {
-  const uint32_t* s = reinterpret_cast<const uint32_t*>(&src);
-  uint32_t* d = reinterpret_cast<uint32_t*>(&p);
-#pragma unroll
-  for (int i = 0; i < (int)(sizeof(TileParams) / 4); ++i) d[i] = (uint32_t)__builtin_amdgcn_readfirstlane((int32_t)s[i]);
+static const BatchKernelSet& pair_kernels(int pair) {
+  static const BatchKernelSet tab[kBPairs] = {batch_kernels_s(),   batch_kernels_d(), batch_kernels_bf16(),
+                                              batch_kernels_f16(), batch_kernels_c(), batch_kernels_z()};
+  return tab[pair];
 }
-
-template <class OP, int VA, int VB, bool BETA>
-__device__ __forceinline__ void run_op_range(const BatchOpDev& o, uint32_t lo, uint32_t hi, unsigned char* lds_raw) {
-  TileWorker<OP, VA, VB, BETA, !BETA> w;
-  TileParams up;
-  uniform_params(o.p, up);
-  w.setup(up, threadIdx.x);
-  float amax = 0.0f;
-  using L = typename OP::L;
-  const typename OP::TA* A = static_cast<const typename OP::TA*>(o.A);
-  typename OP::TB* B = static_cast<typename OP::TB*>(o.B);
-  const typename OP::AL al = (typename OP::AL)o.alpha;
-  const typename OP::BE be = (typename OP::BE)o.beta;
-  w.run_table(up, A, B, reinterpret_cast<L*>(lds_raw), o.bases, lo, hi, al, be, 1.0f, amax);
-}
-
-template <class OP, int VA, int VB, bool BETA>
-__global__ void __launch_bounds__(kThreads) batch_group_kernel(const BatchOpDev* __restrict__ ops, int nops,
-                                                               const uint32_t* __restrict__ wg_begin) {
-  extern __shared__ __align__(16) unsigned char lds_raw[];
-  const uint32_t t0 = wg_begin[blockIdx.x], t1 = wg_begin[blockIdx.x + 1];
-  if (t0 >= t1) return;
-  int lo = 0, hi = nops - 1;  // last op with tile_begin <= t0
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (ops[mid].tile_begin <= t0) lo = mid;
-    else hi = mid - 1;
+static const void* group_kernel(int group) {
+  const BatchKernelSet& k = pair_kernels(group % kBPairs);
+  switch (group_family(group)) {
+    case kGroupRC: return k.rc;
+    case kGroupRow: return k.row;
+    default: return k.box;
   }
-  uint32_t t = t0;
-  for (int op = lo; op < nops && t < t1; ++op) {
-    const BatchOpDev& o = ops[op];
-    const uint32_t end = o.tile_begin + o.ntiles;
-    const uint32_t stop = end < t1 ? end : t1;
-    if (t < stop) {
-      run_op_range<OP, VA, VB, BETA>(o, t - o.tile_begin, stop - o.tile_begin, lds_raw);
-      t = stop;
-    }
-  }
-}
-
-// A row x column op's parameters, read through the constant address space
-// (scalar loads), word by word.
-using KWord = const __attribute__((address_space(4))) uint32_t;
-__device__ __forceinline__ void scalar_rc(KRCOp& o, RCParams& p) {
-  KWord* s = (KWord*)&o.p;
-  uint32_t* d = reinterpret_cast<uint32_t*>(&p);
-#pragma unroll
-  for (int i = 0; i < (int)(sizeof(RCParams) / 4); ++i) d[i] = s[i];
-}
-
-template <class OP, int VA, int VB, bool BETA>
-__global__ void __launch_bounds__(kRcThreads) batch_rc_group_kernel(const BatchRCOpDev* __restrict__ ops, int nops,
-                                                                    const uint32_t* __restrict__ wg_begin) {
-  extern __shared__ __align__(16) unsigned char lds_raw[];
-  const uint32_t t0 = wg_begin[blockIdx.x], t1 = wg_begin[blockIdx.x + 1];
-  if (t0 >= t1) return;
-  int lo = 0, hi = nops - 1;  // last op with tile_begin <= t0
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (ops[mid].tile_begin <= t0) lo = mid;
-    else hi = mid - 1;
-  }
-  KRCOp* kops = (KRCOp*)ops;
-  RCWorker<OP, VA, VB, BETA, false> w;
-  uint32_t t = t0;
-  for (int op = lo; op < nops && t < t1; ++op) {
-    KRCOp& o = kops[op];
-    const uint32_t end = o.tile_begin + o.ntiles;
-    const uint32_t stop = end < t1 ? end : t1;
-    if (t < stop) {
-      RCParams up;
-      scalar_rc(o, up);
-      // ends with a barrier; the previous op's last tile ended with one
-      w.setup_tables(up, o.tables, lds_raw, threadIdx.x);
-      rc_run_table(w, up, (KTile*)o.tiles, t - o.tile_begin, stop - o.tile_begin,
-                   static_cast<const typename OP::TA*>(o.A), static_cast<typename OP::TB*>(o.B),
-                   (typename OP::AL)o.alpha, (typename OP::BE)o.beta);
-      t = stop;
-    }
-  }
-}
-
-// The kinds of one family: ((pair * 2 + wide_a) * 2 + wide_b) * 2 + beta_is_zero.
-template <class OP, int WA, int WB>
-static void add_rc_pair(const void** tab, int pair) {
-  const void* k[8] = {
-      reinterpret_cast<const void*>(&batch_rc_group_kernel<OP, 1, 1, true>),
-      reinterpret_cast<const void*>(&batch_rc_group_kernel<OP, 1, 1, false>),
-      reinterpret_cast<const void*>(&batch_rc_group_kernel<OP, 1, WB, true>),
-      reinterpret_cast<const void*>(&batch_rc_group_kernel<OP, 1, WB, false>),
-      reinterpret_cast<const void*>(&batch_rc_group_kernel<OP, WA, 1, true>),
-      reinterpret_cast<const void*>(&batch_rc_group_kernel<OP, WA, 1, false>),
-      reinterpret_cast<const void*>(&batch_rc_group_kernel<OP, WA, WB, true>),
-      reinterpret_cast<const void*>(&batch_rc_group_kernel<OP, WA, WB, false>),
-  };
-  for (int i = 0; i < 8; ++i) tab[kKindsPerFamily + pair * 8 + i] = k[i];
-}
-
-template <class OP, int WA, int WB>
-static void add_pair(const void** tab, int pair) {
-  const void* k[8] = {
-      reinterpret_cast<const void*>(&batch_group_kernel<OP, 1, 1, true>),
-      reinterpret_cast<const void*>(&batch_group_kernel<OP, 1, 1, false>),
-      reinterpret_cast<const void*>(&batch_group_kernel<OP, 1, WB, true>),
-      reinterpret_cast<const void*>(&batch_group_kernel<OP, 1, WB, false>),
-      reinterpret_cast<const void*>(&batch_group_kernel<OP, WA, 1, true>),
-      reinterpret_cast<const void*>(&batch_group_kernel<OP, WA, 1, false>),
-      reinterpret_cast<const void*>(&batch_group_kernel<OP, WA, WB, true>),
-      reinterpret_cast<const void*>(&batch_group_kernel<OP, WA, WB, false>),
-  };
-  for (int i = 0; i < 8; ++i) tab[pair * 8 + i] = k[i];
-}
-
-struct GroupKernels {
-  const void* k[2 * kKindsPerFamily];
-  GroupKernels() {
-    add_pair<OpF32<float, float>, 4, 4>(k, kBS);
-    add_pair<OpDD, 2, 2>(k, kBD);
-    add_pair<OpF32<bf16s, bf16s>, 8, 8>(k, kBBF16);
-    add_pair<OpF32<f16s, f16s>, 8, 8>(k, kBF16);
-    add_pair<OpCC, 2, 2>(k, kBC);
-    add_pair<OpZZ, 1, 1>(k, kBZ);
-    add_rc_pair<OpF32<float, float>, 4, 4>(k, kBS);
-    add_rc_pair<OpDD, 2, 2>(k, kBD);
-    add_rc_pair<OpF32<bf16s, bf16s>, 8, 8>(k, kBBF16);
-    add_rc_pair<OpF32<f16s, f16s>, 8, 8>(k, kBF16);
-    add_rc_pair<OpCC, 2, 2>(k, kBC);
-    add_rc_pair<OpZZ, 1, 1>(k, kBZ);
-  }
-};
-static bool kind_is_rc(int kind) { return kind >= kKindsPerFamily; }
-static const void* group_kernel(int kind) {
-  static const GroupKernels tab;  // thread-safe initialisation
-  return tab.k[kind];
 }
 
 static int batch_pair(int ta, int tb) {
@@ -257,8 +92,8 @@ using namespace ttc;
 
 // One launch of the grouped execution: the ops of one kind.
 struct BatchGroup {
-  int kind;
-  int op0, nops;   // slice of the op table of the kind's family
+  int group;       // family * kBPairs + type pair
+  int op0, nops;   // slice of the op table of the group's family
   int wg0, nwg;    // slice of the work-group table (nwg + 1 entries)
   int lds_bytes;
 };
@@ -270,9 +105,10 @@ struct BatchGroup {
 // op records point at their tile records once the image's device address is
 // known (render).
 struct BatchImage {
-  enum { kBox, kRC, kWg, kBases, kRcTiles, kRcTables, kSections };
+  enum { kBox, kRC, kRow, kWg, kBases, kRcTiles, kRcTables, kSections };
   std::vector<BatchOpDev> box;
   std::vector<BatchRCOpDev> rcs;
+  std::vector<BatchRowOpDev> rows;
   std::vector<uint32_t> wg;
   std::vector<TileBase> bases;
   std::vector<RCTile> rct;
@@ -283,9 +119,10 @@ struct BatchImage {
   size_t bytes = 0;
 
   void layout() {
-    const size_t len[kSections] = {sizeof(BatchOpDev) * box.size(), sizeof(BatchRCOpDev) * rcs.size(),
-                                   sizeof(uint32_t) * wg.size(),      sizeof(TileBase) * bases.size(),
-                                   sizeof(RCTile) * rct.size(),       sizeof(int32_t) * rtab.size()};
+    const size_t len[kSections] = {sizeof(BatchOpDev) * box.size(),   sizeof(BatchRCOpDev) * rcs.size(),
+                                   sizeof(BatchRowOpDev) * rows.size(), sizeof(uint32_t) * wg.size(),
+                                   sizeof(TileBase) * bases.size(),     sizeof(RCTile) * rct.size(),
+                                   sizeof(int32_t) * rtab.size()};
     size_t o = 0;
     for (int s = 0; s < kSections; ++s) {
       off[s] = o;
@@ -310,6 +147,7 @@ struct BatchImage {
     };
     put(kBox, b2.data(), sizeof(BatchOpDev) * b2.size());
     put(kRC, r2.data(), sizeof(BatchRCOpDev) * r2.size());
+    put(kRow, rows.data(), sizeof(BatchRowOpDev) * rows.size());
     put(kWg, wg.data(), sizeof(uint32_t) * wg.size());
     put(kBases, bases.data(), sizeof(TileBase) * bases.size());
     put(kRcTiles, rct.data(), sizeof(RCTile) * rct.size());
@@ -451,28 +289,22 @@ static ttc_status_t plan_batch(int nops, const ttc_batch_op_t* ops, int wg_per_c
       any_lda |= op.lda[k] != 0;
       any_ldb |= op.ldb[k] != 0;
     }
-    // vector widths: 1 or the widest of the type (the kinds instantiated)
+    // vector widths the group kernels instantiate: 1, 2 and the widest on
+    // row x column tiles and row copies, 1 and the widest on box tiles
     ttc_config_t cfg = op.cfg;
     if (cfg.tile_order > 2) cfg.tile_order = 2;  // grouped launches walk host-built tile records in B order
     if (cfg.lds_buffers == 0) cfg.lds_buffers = 1;  // one LDS buffer unless the op asks for two (measured faster)
-    const int wide = std::max(1, 16 / dtype_bytes(op.type_a));
+    const int wide = pair_kernels(pair).widest;
+    auto width_ok = [wide](const ttc_plan_s* q, int v) {
+      return v == 1 || v == wide || (v == 2 && wide > 2 && q->family != kFamilyBox);
+    };
     for (int pass = 0; pass < 2; ++pass) {
       ttc_plan_t pl = nullptr;
       ttc_status_t st = ttc_plan_create(&pl, op.type_a, op.type_b, op.dim, op.perm, op.size,
                                         any_lda ? op.lda : nullptr, any_ldb ? op.ldb : nullptr,
                                         op.beta_is_zero, &cfg);
-      if (st == TTC_SUCCESS && pl->family == kFamilyRow) {
-        // grouped launches have no row-copy kind: box tiles for this op
-        ttc_plan_destroy(pl);
-        pl = nullptr;
-        cfg.family = kFamilyBox;
-        for (int k = 0; k < TTC_MAX_DIM; ++k) cfg.tile[k] = 0;
-        st = ttc_plan_create(&pl, op.type_a, op.type_b, op.dim, op.perm, op.size, any_lda ? op.lda : nullptr,
-                             any_ldb ? op.ldb : nullptr, op.beta_is_zero, &cfg);
-      }
       if (st != TTC_SUCCESS) return st;
-      const bool ok_a = pl->va == 1 || pl->va == wide;
-      const bool ok_b = pl->vb == 1 || pl->vb == wide;
+      const bool ok_a = width_ok(pl, pl->va), ok_b = width_ok(pl, pl->vb);
       if ((ok_a && ok_b) || pass == 1) {
         bp.plans[i] = pl;
         break;
@@ -483,14 +315,18 @@ static ttc_status_t plan_batch(int nops, const ttc_batch_op_t* ops, int wg_per_c
       for (int k = 0; k < TTC_MAX_DIM; ++k) cfg.tile[k] = 0;  // re-derive the tile for the new widths
     }
     const ttc_plan_s* pl = bp.plans[i];
-    if ((pl->va != 1 && pl->va != wide) || (pl->vb != 1 && pl->vb != wide)) return TTC_ERROR_UNSUPPORTED;
+    if (!width_ok(pl, pl->va) || !width_ok(pl, pl->vb)) return TTC_ERROR_UNSUPPORTED;
     if (pl->g.elements > 0 && (!op.A || !op.B)) return TTC_ERROR_INVALID_VALUE;
     const ttc_plan_s* use = pl->g.elements > 0 ? variant_for(pl, op.A, op.B) : pl;
     if (!use) return TTC_ERROR_INVALID_VALUE;
-    const int wa = use->va > 1 ? 1 : 0, wb = use->vb > 1 ? 1 : 0;
-    const int fam = use->family == kFamilyRC ? 1 : 0;
+    if (!width_ok(use, use->va) || !width_ok(use, use->vb)) return TTC_ERROR_UNSUPPORTED;
+    const int fam = use->family == kFamilyRC ? kGroupRC : (use->family == kFamilyRow ? kGroupRow : kGroupBox);
+    const int bz = pl->beta_is_zero ? 1 : 0;
+    const int ia = index_of_width(use->va, wide), ib = index_of_width(use->vb, wide);
     OpRec& r = rec[i];
-    r.kind = (((fam * kBPairs + pair) * 2 + wa) * 2 + wb) * 2 + (pl->beta_is_zero ? 1 : 0);
+    // a row copy has one width for both sides
+    const int variant = fam == kGroupRow ? ia * 2 + bz : (ia * 3 + ib) * 2 + bz;
+    r.kind = (fam * kBPairs + pair) * kVariantStride + variant;
     r.ntiles = pl->g.elements > 0 ? use->p.ntiles : 0;
     r.src = i;
     const int64_t sa = dtype_bytes(op.type_a), sb = dtype_bytes(op.type_b);
@@ -508,9 +344,11 @@ static ttc_status_t plan_batch(int nops, const ttc_batch_op_t* ops, int wg_per_c
     int e = j;
     uint64_t tiles = 0;
     int lds = 16;
-    const bool rc = kind_is_rc(rec[j].kind);
-    const size_t op0 = rc ? im.rcs.size() : im.box.size();
-    while (e < nops && rec[e].kind == rec[j].kind) {
+    const int group = group_of(rec[j].kind);
+    const int family = group_family(group);
+    const bool rc = family == kGroupRC;
+    const size_t op0 = rc ? im.rcs.size() : (family == kGroupRow ? im.rows.size() : im.box.size());
+    while (e < nops && group_of(rec[e].kind) == group) {
       OpRec& r = rec[e];
       r.tile_begin = (uint32_t)tiles;
       tiles += r.ntiles;
@@ -518,7 +356,19 @@ static ttc_status_t plan_batch(int nops, const ttc_batch_op_t* ops, int wg_per_c
       const ttc_plan_s* pl = bp.plans[r.src];
       const ttc_plan_s* use = pl->g.elements > 0 ? variant_for(pl, op.A, op.B) : pl;
       lds = std::max(lds, use->lds_bytes);
-      if (rc) {
+      if (family == kGroupRow) {
+        BatchRowOpDev d;
+        std::memset(&d, 0, sizeof(d));
+        d.p = use->row;
+        d.A = op.A;
+        d.B = op.B;
+        d.alpha = op.alpha;
+        d.beta = op.beta;
+        d.variant = r.kind % kVariantStride;
+        d.tile_begin = r.tile_begin;
+        d.ntiles = r.ntiles;  // chunks of kThreads * kRowUnroll vectors
+        im.rows.push_back(d);
+      } else if (rc) {
         BatchRCOpDev d;
         std::memset(&d, 0, sizeof(d));
         d.p = use->rc;
@@ -526,6 +376,7 @@ static ttc_status_t plan_batch(int nops, const ttc_batch_op_t* ops, int wg_per_c
         d.B = op.B;
         d.alpha = op.alpha;
         d.beta = op.beta;
+        d.variant = r.kind % kVariantStride;
         d.tile_begin = r.tile_begin;
         d.ntiles = r.ntiles;
         const size_t first = im.rct.size();
@@ -545,7 +396,7 @@ static ttc_status_t plan_batch(int nops, const ttc_batch_op_t* ops, int wg_per_c
         d.B = op.B;
         d.alpha = op.alpha;
         d.beta = op.beta;
-        d.kind = r.kind;
+        d.variant = r.kind % kVariantStride;
         d.tile_begin = r.tile_begin;
         d.ntiles = r.ntiles;
         const size_t first = im.bases.size();
@@ -560,11 +411,11 @@ static ttc_status_t plan_batch(int nops, const ttc_batch_op_t* ops, int wg_per_c
     lds = (lds + 15) / 16 * 16;
     if (tiles > 0) {
       BatchGroup grp;
-      grp.kind = rec[j].kind;
+      grp.group = group;
       grp.op0 = (int)op0;
       grp.nops = e - j;
       grp.lds_bytes = lds;
-      const void* fn = group_kernel(grp.kind);
+      const void* fn = group_kernel(grp.group);
       const int threads = rc ? kRcThreads : kThreads;
       const int occ = occupancy(fn, lds, threads);
       int wg = wg_per_cu > 0 ? wg_per_cu : std::min(occ, 4 * kThreads / threads);
@@ -593,9 +444,17 @@ struct GroupArgs {
 static GroupArgs group_args(const BatchPlan& bp, const BatchGroup& g, uintptr_t base) {
   const BatchImage& im = bp.img;
   GroupArgs a;
-  a.ops = kind_is_rc(g.kind)
-              ? reinterpret_cast<const void*>(base + im.off[BatchImage::kRC] + g.op0 * sizeof(BatchRCOpDev))
-              : reinterpret_cast<const void*>(base + im.off[BatchImage::kBox] + g.op0 * sizeof(BatchOpDev));
+  switch (group_family(g.group)) {
+    case kGroupRC:
+      a.ops = reinterpret_cast<const void*>(base + im.off[BatchImage::kRC] + g.op0 * sizeof(BatchRCOpDev));
+      break;
+    case kGroupRow:
+      a.ops = reinterpret_cast<const void*>(base + im.off[BatchImage::kRow] + g.op0 * sizeof(BatchRowOpDev));
+      break;
+    default:
+      a.ops = reinterpret_cast<const void*>(base + im.off[BatchImage::kBox] + g.op0 * sizeof(BatchOpDev));
+      break;
+  }
   a.nops = g.nops;
   a.wg = reinterpret_cast<const uint32_t*>(base + im.off[BatchImage::kWg] + g.wg0 * sizeof(uint32_t));
   return a;
@@ -605,8 +464,8 @@ static ttc_status_t launch_groups(const BatchPlan& bp, uintptr_t base, hipStream
   for (const BatchGroup& g : bp.groups) {
     GroupArgs a = group_args(bp, g, base);
     void* args[] = {&a.ops, &a.nops, &a.wg};
-    const hipError_t e = hipLaunchKernel(group_kernel(g.kind), dim3(g.nwg),
-                                         dim3(kind_is_rc(g.kind) ? kRcThreads : kThreads), args,
+    const hipError_t e = hipLaunchKernel(group_kernel(g.group), dim3(g.nwg),
+                                         dim3(group_family(g.group) == kGroupRC ? kRcThreads : kThreads), args,
                                          (size_t)g.lds_bytes, s);
     if (e != hipSuccess) return TTC_ERROR_HIP;
   }
@@ -679,9 +538,9 @@ extern "C" ttc_status_t ttc_batch_graph_launch(ttc_batch_t b, ttc_stream_t strea
         void* args[] = {&a.ops, &a.nops, &a.wg};
         hipKernelNodeParams kp;
         std::memset(&kp, 0, sizeof(kp));
-        kp.func = const_cast<void*>(group_kernel(gr.kind));
+        kp.func = const_cast<void*>(group_kernel(gr.group));
         kp.gridDim = dim3(gr.nwg);
-        kp.blockDim = dim3(kind_is_rc(gr.kind) ? kRcThreads : kThreads);
+        kp.blockDim = dim3(group_family(gr.group) == kGroupRC ? kRcThreads : kThreads);
         kp.sharedMemBytes = (unsigned)gr.lds_bytes;
         kp.kernelParams = args;
         hipGraphNode_t node;
