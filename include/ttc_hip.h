@@ -192,7 +192,10 @@ typedef struct {
   ttc_config_t cfg;           /* per-op tile choice; zeros = heuristic */
 } ttc_batch_op_t;
 
-/* Creates the work table on the current device (allocates device memory). */
+/* Creates the work table on the current device (allocates device memory).
+ * wg_per_cu: work-group slices per CU in each group's grid (residency is
+ * set by occupancy; more slices than resident work-groups balance uneven
+ * ops); 0 = 96. */
 ttc_status_t ttc_batch_create(ttc_batch_t* batch, int nops, const ttc_batch_op_t* ops, int wg_per_cu);
 /* One grouped launch of every op. */
 ttc_status_t ttc_batch_execute(ttc_batch_t batch, ttc_stream_t stream);
