@@ -47,12 +47,16 @@ def log(*a):
 
 def kernel_name(info):
     """The kernel a plan launches, spelled as rocprof shows its template
-    (ttc::<family kernel><Op, VA, VB, BETA, NT>)."""
+    (ttc::<family kernel><Op, VA, VB, BETA, NT[, FAST]>)."""
     ops = {(0, 0): "ttc::OpF32<float, float>", (1, 1): "ttc::OpDD"}
-    fam = "transpose_rc_kernel" if info["family"] == 2 else "transpose_kernel"
-    return "ttc::%s<%s, %d, %d, %s, %s>" % (fam, ops.get((info["type_a"], info["type_b"]), "?"), info["vec_a"],
-                                            info["vec_b"], "false" if info["beta_is_zero"] else "true",
-                                            "true" if info["nt_store"] else "false")
+    tf = lambda x: "true" if x else "false"
+    op = ops.get((info["type_a"], info["type_b"]), "?")
+    if info["family"] == 2:
+        return "ttc::transpose_rc_kernel<%s, %d, %d, %s, %s, %s>" % (
+            op, info["vec_a"], info["vec_b"], tf(not info["beta_is_zero"]), tf(info["nt_store"]),
+            tf(info["lane_fixed"]))
+    return "ttc::transpose_kernel<%s, %d, %d, %s, %s>" % (op, info["vec_a"], info["vec_b"],
+                                                        tf(not info["beta_is_zero"]), tf(info["nt_store"]))
 
 
 def build_c2(torch, rt, wl, dev, tune=0):

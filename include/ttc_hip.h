@@ -115,6 +115,7 @@ typedef struct {
   int rows, cols;                /* row x column tile: row and column counts (0 for box tiles) */
   int tile_order;                /* 1 = A order, 2 = B order, 3 / 4 grouped B order */
   int lds_buffers;               /* LDS tile buffers of the row x column kernel (1 for other families) */
+  int lane_fixed;                /* row x column kernel: full tiles take the lane-fixed walk (1) or the general one */
 } ttc_plan_info_t;
 
 /* Library version string. */

@@ -148,6 +148,35 @@ def test_explicit_tiles_with_two_partial_dims(gpu):
                                                               vec_b=1 if tile[1] % 2 else 2))
 
 
+def test_lane_fixed_walk(gpu):
+    """Row x column launches of >= 2048 tiles with a 2-byte side take the
+    lane-fixed walk on full tiles (the general walk on partial ones): tiles
+    with exact powers of two and with up to 1/16 padding on either walk,
+    every vector width pair the kernels have, beta read or not, partial last
+    tiles on both sides and padded leading dimensions, all bit-exact."""
+    cases = [  # (tA, tB, perm, size, tile, vec pairs)
+        (BF16, BF16, [1, 0], [4096, 4104], [64, 128], [(1, 1), (8, 8), (2, 8), (8, 1)]),
+        (BF16, BF16, [1, 0], [4000, 4100], [62, 128], [(1, 1), (2, 2)]),   # rows 62 -> 64
+        (S, BF16, [1, 0], [2048, 8200], [128, 64], [(4, 8), (1, 1), (4, 2)]),
+        (BF16, S, [1, 0], [1920, 9000], [124, 62], [(2, 2), (1, 1)]),      # rows 124 -> 128, cols 62 -> 64
+        (F16, F16, [2, 1, 0], [16, 128, 4200], [16, 4, 64], [(8, 8), (1, 2)]),  # rows: two dimensions
+    ]
+    n = 0
+    for i, (tA, tB, perm, size, tile, vecs) in enumerate(cases):
+        for va, vb in vecs:
+            cfg = dict(family=2, tile=tile, vec_a=va, vec_b=vb, lds_buffers=1 + n % 2)
+            for beta, bz in ((0.0, True), (0.75, False)):
+                info = check(gpu, tA, tB, perm, size, 1.25, beta, seed=i, dist="normal", config=cfg, beta_is_zero=bz)
+                assert info["lane_fixed"] == 1 and info["tiles"] >= 2048, info
+                n += 1
+        # padded leading dimensions
+        lda = [s + 3 for s in size]
+        ldb = [size[perm[q]] + 5 for q in range(len(size))]
+        info = check(gpu, tA, tB, perm, size, 0.5, 1.0, lda, ldb, seed=i, config=dict(family=2, tile=tile))
+        assert info["lane_fixed"] == 1, info
+    assert n >= 26
+
+
 def test_explicit_row_column_tiles(gpu):
     """Row x column tiles with partial last tiles on the row group, the
     column group or both, multi-dimension groups and non-dense tables."""

@@ -206,35 +206,6 @@ static void fill_rc_tiles(const RCParams& p, RCTile* out) {
   }
 }
 
-// The offset tables of a row x column op, as the kernel's setup() builds
-// them: rowA[R], rowB[R], colA[C], colB[C] (byte offsets).
-static void rc_tables(const RCParams& p, int32_t* out) {
-  for (int i = 0; i < p.R; ++i) {
-    uint32_t l = (uint32_t)i;
-    int32_t oa = 0, ob = 0;
-    for (int m = 0; m < p.na; ++m) {
-      const uint32_t d = l % (uint32_t)p.ta[m];
-      l /= (uint32_t)p.ta[m];
-      oa += (int32_t)d * p.saA[m];
-      ob += (int32_t)d * p.sbA[m];
-    }
-    out[i] = oa;
-    out[p.R + i] = ob;
-  }
-  for (int j = 0; j < p.C; ++j) {
-    uint32_t l = (uint32_t)j;
-    int32_t oa = 0, ob = 0;
-    for (int m = 0; m < p.nb; ++m) {
-      const uint32_t d = l % (uint32_t)p.tb[m];
-      l /= (uint32_t)p.tb[m];
-      oa += (int32_t)d * p.saB[m];
-      ob += (int32_t)d * p.sbB[m];
-    }
-    out[2 * p.R + j] = oa;
-    out[2 * p.R + p.C + j] = ob;
-  }
-}
-
 // One op of the work table while it is built.
 struct OpRec {
   int kind;
@@ -374,6 +345,14 @@ static ttc_status_t plan_batch(int nops, const ttc_batch_op_t* ops, int wg_per_c
         BatchRCOpDev d;
         std::memset(&d, 0, sizeof(d));
         d.p = use->rc;
+        if (d.p.fast) {
+          // the grouped kernels run the general walk (its LDS region holds
+          // the lane-fixed walk's, so the plan's LDS size still covers it)
+          d.p.fast = 0;
+          d.p.Rw = d.p.R;
+          d.p.Cw = d.p.C;
+          d.p.ntab = 2 * d.p.R + 2 * d.p.C;
+        }
         d.A = op.A;
         d.B = op.B;
         d.alpha = op.alpha;
@@ -385,8 +364,8 @@ static ttc_status_t plan_batch(int nops, const ttc_batch_op_t* ops, int wg_per_c
         im.rct.resize(first + r.ntiles);
         if (r.ntiles) fill_rc_tiles(use->rc, im.rct.data() + first);
         const size_t tab = im.rtab.size();
-        im.rtab.resize(tab + 2 * (size_t)use->rc.R + 2 * (size_t)use->rc.C);
-        if (r.ntiles) rc_tables(use->rc, im.rtab.data() + tab);
+        im.rtab.resize(tab + (size_t)d.p.ntab);
+        if (r.ntiles) rc_fill_tables(d.p, im.rtab.data() + tab);
         im.rc_rec.push_back(first);
         im.rc_tab.push_back(tab);
         im.rcs.push_back(d);

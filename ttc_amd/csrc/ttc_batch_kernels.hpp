@@ -60,7 +60,7 @@ struct BatchRCOpDev {
   uint32_t tile_begin;    // first tile of this op in its group
   uint32_t ntiles;
   const RCTile* tiles;    // this op's tile records
-  const int32_t* tables;  // its offset tables: rowA[R], rowB[R], colA[C], colB[C]
+  const int32_t* tables;  // its LDS offset tables (rc_fill_tables: RCParams::ntab entries)
 };
 
 // Op record of a row-copy group (perm[0] == 0 after fusion, dimension 0

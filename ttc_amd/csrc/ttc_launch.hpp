@@ -34,6 +34,10 @@ struct KernelEntry {
   const void* func;
   LaunchRowFn launch_row;
   LaunchFp8RCFn launch_fp8_rc;
+  // row x column family: the instance for RCParams::fast launches
+  LaunchRCFn launch_rc_fast;
+  LaunchFp8RCFn launch_fp8_rc_fast;
+  const void* func_fast;
 };
 
 template <class OP, int VA, int VB, bool BETA, bool NT>
@@ -59,18 +63,18 @@ hipError_t launch_fp8(const TileParams& p, const void* A, void* B, float alpha, 
                          args, lds, s);
 }
 
-template <class OP, int VA, int VB, bool BETA, bool NT>
+template <class OP, int VA, int VB, bool BETA, bool NT, bool FAST>
 hipError_t launch_rc(const RCParams& p, const void* A, void* B, double alpha, double beta, unsigned grid, size_t lds,
                      hipStream_t s) {
   const typename OP::TA* a = static_cast<const typename OP::TA*>(A);
   typename OP::TB* b = static_cast<typename OP::TB*>(B);
   RCParams pp = p;
   void* args[] = {&pp, &a, &b, &alpha, &beta};
-  return hipLaunchKernel(reinterpret_cast<const void*>(&transpose_rc_kernel<OP, VA, VB, BETA, NT>), dim3(grid),
+  return hipLaunchKernel(reinterpret_cast<const void*>(&transpose_rc_kernel<OP, VA, VB, BETA, NT, FAST>), dim3(grid),
                          dim3(kRcThreads), args, lds, s);
 }
 
-template <class OP, int VA, int VB>
+template <class OP, int VA, int VB, bool FAST>
 hipError_t launch_fp8_rc(const RCParams& p, const void* A, void* B, float alpha, const float* scale_in,
                          const float* amax_in, float fp8_max, float scale_host, float* amax_out, float* scale_inv_out,
                          unsigned grid, size_t lds, hipStream_t s) {
@@ -78,7 +82,7 @@ hipError_t launch_fp8_rc(const RCParams& p, const void* A, void* B, float alpha,
   typename OP::TB* b = static_cast<typename OP::TB*>(B);
   RCParams pp = p;
   void* args[] = {&pp, &a, &b, &alpha, &scale_in, &amax_in, &fp8_max, &scale_host, &amax_out, &scale_inv_out};
-  return hipLaunchKernel(reinterpret_cast<const void*>(&cast_fp8_rc_kernel<OP, VA, VB>), dim3(grid),
+  return hipLaunchKernel(reinterpret_cast<const void*>(&cast_fp8_rc_kernel<OP, VA, VB, FAST>), dim3(grid),
                          dim3(kRcThreads), args, lds, s);
 }
 
@@ -86,13 +90,17 @@ template <class OP, int VA, int VB, int MODE>
 KernelEntry make_rc_entry(int ta, int tb) {
   KernelEntry e{ta, tb, VA, VB, MODE, kFamRC, nullptr, nullptr, nullptr, nullptr};
   if constexpr (MODE == kModeFp8) {
-    e.launch_fp8_rc = &launch_fp8_rc<OP, VA, VB>;
-    e.func = reinterpret_cast<const void*>(&cast_fp8_rc_kernel<OP, VA, VB>);
+    e.launch_fp8_rc = &launch_fp8_rc<OP, VA, VB, false>;
+    e.func = reinterpret_cast<const void*>(&cast_fp8_rc_kernel<OP, VA, VB, false>);
+    e.launch_fp8_rc_fast = &launch_fp8_rc<OP, VA, VB, true>;
+    e.func_fast = reinterpret_cast<const void*>(&cast_fp8_rc_kernel<OP, VA, VB, true>);
   } else {
     constexpr bool beta = MODE == kModeBeta;
     constexpr bool nt = MODE == kModeBzNt;
-    e.launch_rc = &launch_rc<OP, VA, VB, beta, nt>;
-    e.func = reinterpret_cast<const void*>(&transpose_rc_kernel<OP, VA, VB, beta, nt>);
+    e.launch_rc = &launch_rc<OP, VA, VB, beta, nt, false>;
+    e.func = reinterpret_cast<const void*>(&transpose_rc_kernel<OP, VA, VB, beta, nt, false>);
+    e.launch_rc_fast = &launch_rc<OP, VA, VB, beta, nt, true>;
+    e.func_fast = reinterpret_cast<const void*>(&transpose_rc_kernel<OP, VA, VB, beta, nt, true>);
   }
   return e;
 }

@@ -82,7 +82,7 @@ struct RCParams {
   int32_t R, C;          // rows (A-group volume) and columns (B-group volume) of a full tile
   int32_t Rl, Cl;        // rows / columns of the last tile along the split dims (== R / C if none)
   FastDiv dR, dRl, dC, dCl;
-  int32_t RP;            // LDS pitch of a column (>= R, odd)
+  int32_t RP;            // LDS pitch of a column (R | 1)
   int32_t rowA_dense;    // rowA[i] == i * sizeof(TA): the rows are one contiguous run of A
   int32_t colB_dense;    // colB[j] == j * sizeof(TB): the columns are one contiguous run of B
   int32_t na, nb;        // dimensions in the A group / B group
@@ -107,9 +107,34 @@ struct RCParams {
   int32_t nbuf;            // LDS tile buffers: 2 lets a tile's staging overlap the previous tile's reads
   uint32_t ntiles;         // grid size (with the padding of a grouped walk)
   int32_t grid_order;
-  int32_t lds_data_elems;  // tile data region in LDS elements (RP * C, a spare vector slot, 16-byte rounding)
+  int32_t lds_data_elems;  // tile data region in LDS elements (both walks' reach, a spare vector slot, 16-byte rounding)
   int32_t lds_spare;       // RP * C: where lanes past a partial tile stage their (unused) values
+  // Lane-fixed walk of full tiles (fast != 0).  The read walk runs over Rw
+  // rows (R rounded up to a power of two that divides the walk step 256 *
+  // VA), so a lane keeps one row i for the whole tile and its slot `it`
+  // takes column j0 + it * kA; the write walk mirrors it over Cw columns
+  // with row step kB.  Positions past R or C read offset tables holding
+  // kRcTabOOB, which sends the access out of the buffer range, and stage
+  // into the spare slot (lds_spare), so no per-slot bounds test remains.
+  // Partial and empty tiles take the general walk.
+  int32_t fast;
+  int32_t Rw, Cw;          // walk extents (powers of two >= R, C); R, C when fast == 0
+  int32_t Rw_shift, Cw_shift;
+  int32_t kA, kB;          // column step of the read walk, row step of the write walk
+  int32_t nia, nib;        // slots per lane of the read and write walks (rc_elems / vector width)
+  int32_t ntab;            // offset-table entries in LDS (layout below)
+  int32_t pad_;
 };
+
+// LDS offset tables of a row x column launch, int32 byte offsets after the
+// tile data: rowA[Rw] rowB[R] colA[C] colB[Cw], then (fast walk only)
+// colS[kA * NIA] and rowS[kB * NIB], the read walk's column offsets and the
+// write walk's row offsets in slot order (colS[jl * NIA + it] = colA[jl + it
+// * kA]); entries past R or C hold kRcTabOOB.  A table entry is below 2^30 -
+// 64, so the sum of two entries is either a real offset (< 2^31) or, if
+// either is kRcTabOOB, in [2^31, 2^32): past the buffer range 2^31.
+constexpr uint32_t kRcTabOOB = 0xC0000000u;
+constexpr long long kRcFastSpan = (1ll << 30) - 64;
 
 // Row copy for perm[0] == 0, where A and B share the stride-1 dimension:
 // a row is one position of the other dimensions, the rows are walked in B

@@ -476,19 +476,26 @@ ttc_status_t build_params(const Geometry& g, const int* tile, int ta, int tb, in
 // LDS per work-group (tile data plus offset tables): four 256-thread
 // work-groups per CU (160 KiB of LDS).
 static constexpr int kRcLdsBudget = 40960 * (kRcThreads / 256);
+static constexpr long long kRcFastTiles = 2048;  // fewest tiles of a launch on the lane-fixed walk
 
 int max_rc_volume(int ta, int tb) {
   const int v = kRcThreads * std::min(rc_elems(dtype_bytes(ta)), rc_elems(dtype_bytes(tb)));
   return std::min(v, 32768 * (kRcThreads / 256) / lds_elem_bytes(ta, tb));
 }
 
-// Tile data (odd pitch RP: the write phase reads LDS with stride RP) plus a
-// spare vector slot, then the four offset tables.
-static long long rc_data_bytes(long long R, long long C, int lds_elem) {
-  return (((R | 1) * C + 16) * lds_elem + 15) / 16 * 16;
-}
-static long long rc_lds(long long R, long long C, int lds_elem, int nbuf = 1) {
-  return nbuf * rc_data_bytes(R, C, lds_elem) + (2 * R + 2 * C) * 4;
+// LDS of a row x column tile: the data region (odd column pitch RP = R | 1,
+// since the write phase reads LDS with stride RP; everything either walk can
+// reach, with a spare vector slot at RP * C where padding slots stage) in
+// bytes, and the offset-table entries.  The lane-fixed write walk reads rows
+// up to capB / Cw - 1 of its lane's column (values of padding slots are never
+// stored), which can reach a little past RP * C.
+static void rc_layout(long long R, long long C, long long Rw, long long Cw, long long capA, long long capB,
+                      int lds_elem, bool fast, long long& RP, long long& data, long long& tab) {
+  RP = R | 1;
+  long long reach = RP * C + 16;
+  if (fast) reach = std::max(reach, capB / Cw + (C - 1) * RP);
+  data = (reach * lds_elem + 15) / 16 * 16;
+  tab = fast ? Rw + R + C + Cw + capA / Rw + capB / Cw : 2 * R + 2 * C;
 }
 
 // Rows: A's leading dimensions 0, 1, ... up to B's stride-1 dimension;
@@ -600,8 +607,33 @@ ttc_status_t build_rc_params(const Geometry& g, const int* tile, int ta, int tb,
   }
   if (R * C > max_rc_volume(ta, tb)) return TTC_ERROR_UNSUPPORTED;
   if (nbuf < 1 || nbuf > 2) return TTC_ERROR_INVALID_VALUE;
-  if (rc_lds(R, C, lds_elem_bytes(ta, tb)) > kRcLdsBudget) return TTC_ERROR_UNSUPPORTED;
-  const long long lds = rc_lds(R, C, lds_elem_bytes(ta, tb), nbuf);
+  // Lane-fixed walk of full tiles: the walks' row / column extents rounded
+  // up to powers of two must divide the walk steps and still cover the tile
+  // within each side's slots, and table entries must stay below kRcFastSpan.
+  // Padding lanes cost memory instructions: at most 1/16 of either walk may
+  // be padding (measured: rows 192 -> 256 or 100 -> 128 lost 8-10%).  Its
+  // longer setup pays off only over a few tiles per work-group: launches of
+  // fewer than kRcFastTiles tiles keep the general walk.  It pays where the
+  // general walk's per-element index arithmetic weighs most, with 2-byte
+  // elements on a side (same-box A/B: fp32 -> bf16 (3,1,0,2) +7%, bf16 ->
+  // fp8 8192 x 14336 +7%); on 4- and 8-byte pairs it measured neutral over
+  // the 64 ops of the 200 MiB fp32/fp64 set and -3% on fp32 -> fp8.
+  const long long capA = (long long)kRcThreads * rc_elems(szA), capB = (long long)kRcThreads * rc_elems(szB);
+  long long Rw = 1, Cw = 1;
+  while (Rw < R) Rw <<= 1;
+  while (Cw < C) Cw <<= 1;
+  bool fast = Rw <= (long long)kRcThreads * va && Cw <= (long long)kRcThreads * vb && Rw * C <= capA &&
+              R * Cw <= capB && spanA * szA < kRcFastSpan && spanB * szB < kRcFastSpan && 16 * Rw <= 17 * R &&
+              16 * Cw <= 17 * C && ntiles >= kRcFastTiles && (szA == 2 || szB == 2);
+  const int le = lds_elem_bytes(ta, tb);
+  long long RP = 0, data = 0, tab = 0;
+  rc_layout(R, C, Rw, Cw, capA, capB, le, fast, RP, data, tab);
+  if (fast && data + tab * 4 > kRcLdsBudget) {
+    fast = false;
+    rc_layout(R, C, Rw, Cw, capA, capB, le, fast, RP, data, tab);
+  }
+  if (data + tab * 4 > kRcLdsBudget) return TTC_ERROR_UNSUPPORTED;
+  const long long lds = nbuf * data + tab * 4;
   const int ka = ga[na - 1], kb = gb[nb - 1];
   const bool partA = g.size[ka] > 0 && g.size[ka] % tile[ka];
   const bool partB = g.size[kb] > 0 && g.size[kb] % tile[kb];
@@ -613,7 +645,17 @@ ttc_status_t build_rc_params(const Geometry& g, const int* tile, int ta, int tb,
   p.dRl = make_fastdiv((uint32_t)p.Rl);
   p.dC = make_fastdiv((uint32_t)p.C);
   p.dCl = make_fastdiv((uint32_t)p.Cl);
-  p.RP = (int32_t)(R | 1);
+  p.RP = (int32_t)RP;
+  p.fast = fast ? 1 : 0;
+  p.Rw = (int32_t)(fast ? Rw : R);
+  p.Cw = (int32_t)(fast ? Cw : C);
+  p.Rw_shift = fast ? __builtin_ctzll((unsigned long long)Rw) : 0;
+  p.Cw_shift = fast ? __builtin_ctzll((unsigned long long)Cw) : 0;
+  p.kA = fast ? (int32_t)((long long)kRcThreads * va / Rw) : 0;
+  p.kB = fast ? (int32_t)((long long)kRcThreads * vb / Cw) : 0;
+  p.nia = rc_elems(szA) / va;
+  p.nib = rc_elems(szB) / vb;
+  p.ntab = (int32_t)tab;
   // Tile grid: dimensions with more than one tile, in A order (1) or B
   // order (2).  Grouped B order (3: groups of 8, 4: groups of 16): the row
   // group's outermost tiled dimension is split into an inner position,
@@ -679,11 +721,62 @@ ttc_status_t build_rc_params(const Geometry& g, const int* tile, int ta, int tb,
   if (grid > INT_MAX) return TTC_ERROR_UNSUPPORTED;
   p.ntiles = (uint32_t)grid;
   p.grid_order = kernel_grid_order(grid_order);
-  p.lds_data_elems = (int32_t)(rc_data_bytes(R, C, lds_elem_bytes(ta, tb)) / lds_elem_bytes(ta, tb));
+  p.lds_data_elems = (int32_t)(data / le);
   p.lds_spare = (int32_t)(p.RP * C);
   p.nbuf = nbuf;
   lds_bytes = (int)lds;
   return TTC_SUCCESS;
+}
+
+// Offsets of tile row i (A and B) and tile column j: the mixed-radix
+// decomposition the kernel's setup() performs.
+static void rc_row(const RCParams& p, int i, int32_t& oa, int32_t& ob) {
+  oa = ob = 0;
+  for (int m = 0; m < p.na; ++m) {
+    const int d = i % p.ta[m];
+    i /= p.ta[m];
+    oa += d * p.saA[m];
+    ob += d * p.sbA[m];
+  }
+}
+static void rc_col(const RCParams& p, int j, int32_t& oa, int32_t& ob) {
+  oa = ob = 0;
+  for (int m = 0; m < p.nb; ++m) {
+    const int d = j % p.tb[m];
+    j /= p.tb[m];
+    oa += d * p.saB[m];
+    ob += d * p.sbB[m];
+  }
+}
+
+void rc_fill_tables(const RCParams& p, int32_t* out) {
+  int32_t* rA = out;
+  int32_t* rB = rA + p.Rw;
+  int32_t* cA = rB + p.R;
+  int32_t* cB = cA + p.C;
+  for (int i = 0; i < p.Rw; ++i) {
+    int32_t oa = (int32_t)kRcTabOOB, ob = 0;
+    if (i < p.R) rc_row(p, i, oa, ob);
+    rA[i] = oa;
+    if (i < p.R) rB[i] = ob;
+  }
+  for (int j = 0; j < p.Cw; ++j) {
+    int32_t oa = 0, ob = (int32_t)kRcTabOOB;
+    if (j < p.C) rc_col(p, j, oa, ob);
+    if (j < p.C) cA[j] = oa;
+    cB[j] = ob;
+  }
+  if (!p.fast) return;
+  int32_t* cS = cB + p.Cw;
+  int32_t* rS = cS + p.kA * p.nia;
+  for (int e = 0; e < p.kA * p.nia; ++e) {
+    const int j = e / p.nia + (e % p.nia) * p.kA;
+    cS[e] = j < p.C ? cA[j] : (int32_t)kRcTabOOB;
+  }
+  for (int e = 0; e < p.kB * p.nib; ++e) {
+    const int i = e / p.nib + (e % p.nib) * p.kB;
+    rS[e] = i < p.R ? rB[i] : (int32_t)kRcTabOOB;
+  }
 }
 
 // Row copy: rows are the positions of dimensions 1..rank-1 in B order
@@ -1063,6 +1156,7 @@ extern "C" ttc_status_t ttc_plan_get_info(ttc_plan_t pl, ttc_plan_info_t* info) 
     info->cols = pl->rc.C;
   }
   info->lds_buffers = pl->family == kFamilyRC ? pl->rc.nbuf : 1;
+  info->lane_fixed = pl->family == kFamilyRC ? pl->rc.fast : 0;
   if (pl->family == kFamilyRow) {
     info->slots_a = info->slots_b = kRowUnroll;
     info->tiles = g.size[0] > 0 ? g.elements / g.size[0] : 0;  // rows

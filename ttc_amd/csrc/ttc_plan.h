@@ -95,6 +95,9 @@ ttc_status_t build_rc_params(const Geometry& g, const int* tile, int ta, int tb,
                              int tile_order, RCParams& p, int& lds_bytes, int nbuf = 1);
 bool choose_rc_tile(const Geometry& g, int ta, int tb, int va, int vb, int run_a, int run_b, int* tile,
                     double* score);
+// The launch's LDS offset tables (p.ntab entries, the layout of RCParams),
+// as the kernel's setup() builds them.
+void rc_fill_tables(const RCParams& p, int32_t* out);
 
 // Row-copy family (perm[0] == 0): launch parameters for vectors of v
 // elements; TTC_ERROR_UNSUPPORTED when the shape does not fit the kernel.

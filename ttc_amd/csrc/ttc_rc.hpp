@@ -31,7 +31,10 @@ __device__ __forceinline__ int64_t uniform64(int64_t v) {
   return (int64_t)(((uint64_t)hi << 32) | lo);
 }
 
-template <class OP, int VA, int VB, bool BETA, bool NT>
+// FAST: the instance that runs full tiles of a p.fast launch on the
+// lane-fixed walk (partial tiles on the general one); the general instance
+// carries none of its code or state.
+template <class OP, int VA, int VB, bool BETA, bool NT, bool FAST = false>
 struct RCWorker {
   using TA = typename OP::TA;
   using TB = typename OP::TB;
@@ -50,6 +53,16 @@ struct RCWorker {
   const int32_t* colA;
   const int32_t* colB;
   uint32_t tid;
+  // Lane-fixed walk (full tiles of a launch with p.fast): this lane's row
+  // offset and slot-ordered column offsets for the read walk, its column
+  // offset and slot-ordered row offsets for the write walk, and the LDS
+  // index of its slot 0 and the step between slots on each side.
+  int32_t fa_row;
+  const int32_t* fa_col;
+  uint32_t fa_lds, fa_step;
+  int32_t fb_col;
+  const int32_t* fb_row;
+  uint32_t fb_lds, fb_step;
 
   // One tile: origin, shape, and each walk's start and per-slot step.  The
   // read walk visits (i, j) with i fastest, the write walk with j fastest;
@@ -57,6 +70,7 @@ struct RCWorker {
   // is an add with one wrap ((i, j) never needs a division after the start).
   struct Tile {
     int64_t bA, bB;        // element offsets of the tile origin
+    bool fast;             // full tile of a fast launch: lane-fixed walk (the fields below are unset)
     uint32_t rows, cols;
     int32_t limA, limB;    // slot it is inside the tile iff it * S < lim (per lane)
     uint32_t ai, aj;       // read walk start (per lane)
@@ -65,45 +79,107 @@ struct RCWorker {
     uint32_t bdi, bdj;     // SB = bdi * cols + bdj
   };
 
-  // Offset tables into LDS (after the data region), once per launch.
+  // LDS table layout (see kRcTabOOB): rowA[Rw] rowB[R] colA[C] colB[Cw]
+  // colS[kA * NIA] rowS[kB * NIB].
+  __device__ __forceinline__ static int32_t* tables(const RCParams& p, unsigned char* lds_raw) {
+    return reinterpret_cast<int32_t*>(lds_raw + (size_t)p.nbuf * (size_t)p.lds_data_elems * sizeof(L));
+  }
+
+  // Offsets of tile row i (A, B) and tile column j (A, B): mixed-radix
+  // digits over the group's tile extents times their byte strides.
+  __device__ __forceinline__ static void row_offsets(const RCParams& p, uint32_t l, int32_t& oa, int32_t& ob) {
+    oa = ob = 0;
+    for (int m = 0; m < p.na; ++m) {
+      const uint32_t q = fdiv(l, p.dta[m]);
+      const uint32_t d = l - q * (uint32_t)p.ta[m];
+      l = q;
+      oa += (int32_t)d * p.saA[m];
+      ob += (int32_t)d * p.sbA[m];
+    }
+  }
+  __device__ __forceinline__ static void col_offsets(const RCParams& p, uint32_t l, int32_t& oa, int32_t& ob) {
+    oa = ob = 0;
+    for (int m = 0; m < p.nb; ++m) {
+      const uint32_t q = fdiv(l, p.dtb[m]);
+      const uint32_t d = l - q * (uint32_t)p.tb[m];
+      l = q;
+      oa += (int32_t)d * p.saB[m];
+      ob += (int32_t)d * p.sbB[m];
+    }
+  }
+
+  // Table pointers and this lane's constants of the lane-fixed walk
+  // (fa_row and fb_col are set by the caller).
+  __device__ __forceinline__ void bind_tables(const RCParams& p, int32_t* tab, uint32_t& ai, uint32_t& bj) {
+    rowA = tab;
+    rowB = tab + p.Rw;
+    colA = rowB + p.R;
+    colB = colA + p.C;
+    if constexpr (!FAST) return;
+    const int32_t* cS = colB + p.Cw;
+    const int32_t* rS = cS + p.kA * NIA;
+    const uint32_t la = tid * VA, lb = tid * VB;
+    ai = la & (uint32_t)(p.Rw - 1);
+    bj = lb & (uint32_t)(p.Cw - 1);
+    const uint32_t jl = la >> p.Rw_shift, il = lb >> p.Cw_shift;
+    // a padding row stages every slot into the spare slot; a padding column
+    // reads element 0 (its stores are dropped)
+    const bool row_ok = ai < (uint32_t)p.R, col_ok = bj < (uint32_t)p.C;
+    fa_col = cS + jl * NIA;
+    fa_lds = row_ok ? ai + jl * (uint32_t)p.RP : (uint32_t)p.lds_spare;
+    fa_step = row_ok ? (uint32_t)p.kA * (uint32_t)p.RP : 0u;
+    fb_row = rS + il * NIB;
+    fb_lds = col_ok ? il + bj * (uint32_t)p.RP : 0u;
+    fb_step = col_ok ? (uint32_t)p.kB : 0u;
+  }
+
+  // Offset tables into LDS (after the data region), once per launch, in
+  // one pass and one barrier (layout: see kRcTabOOB).
   __device__ __forceinline__ void setup(const RCParams& p, unsigned char* lds_raw, uint32_t tid_) {
     tid = tid_;
     data = reinterpret_cast<L*>(lds_raw);
-    int32_t* tab = reinterpret_cast<int32_t*>(lds_raw + (size_t)p.nbuf * (size_t)p.lds_data_elems * sizeof(L));
+    int32_t* tab = tables(p, lds_raw);
     int32_t* rA = tab;
-    int32_t* rB = tab + p.R;
-    int32_t* cA = tab + 2 * p.R;
-    int32_t* cB = tab + 2 * p.R + p.C;
-    for (int i = (int)tid; i < p.R; i += kRcThreads) {
-      uint32_t l = (uint32_t)i;
-      int32_t oa = 0, ob = 0;
-      for (int m = 0; m < p.na; ++m) {
-        const uint32_t q = fdiv(l, p.dta[m]);
-        const uint32_t d = l - q * (uint32_t)p.ta[m];
-        l = q;
-        oa += (int32_t)d * p.saA[m];
-        ob += (int32_t)d * p.sbA[m];
-      }
+    int32_t* rB = tab + p.Rw;
+    int32_t* cA = rB + p.R;
+    int32_t* cB = cA + p.C;
+    // (the general instance needs no padding entries)
+    const int nr = FAST ? p.Rw : p.R, nc = FAST ? p.Cw : p.C;
+    for (int i = (int)tid; i < nr; i += kRcThreads) {
+      int32_t oa = (int32_t)kRcTabOOB, ob = 0;
+      if (i < p.R) row_offsets(p, (uint32_t)i, oa, ob);
       rA[i] = oa;
-      rB[i] = ob;
+      if (i < p.R) rB[i] = ob;
     }
-    for (int j = (int)tid; j < p.C; j += kRcThreads) {
-      uint32_t l = (uint32_t)j;
-      int32_t oa = 0, ob = 0;
-      for (int m = 0; m < p.nb; ++m) {
-        const uint32_t q = fdiv(l, p.dtb[m]);
-        const uint32_t d = l - q * (uint32_t)p.tb[m];
-        l = q;
-        oa += (int32_t)d * p.saB[m];
-        ob += (int32_t)d * p.sbB[m];
-      }
-      cA[j] = oa;
+    for (int j = (int)tid; j < nc; j += kRcThreads) {
+      int32_t oa = 0, ob = (int32_t)kRcTabOOB;
+      if (j < p.C) col_offsets(p, (uint32_t)j, oa, ob);
+      if (j < p.C) cA[j] = oa;
       cB[j] = ob;
     }
-    rowA = rA;
-    rowB = rB;
-    colA = cA;
-    colB = cB;
+    uint32_t ai = 0, bj = 0;
+    bind_tables(p, tab, ai, bj);
+    if (FAST && p.fast) {
+      int32_t* cS = cB + p.Cw;
+      int32_t* rS = cS + p.kA * NIA;
+      for (int e = (int)tid; e < p.kA * NIA; e += kRcThreads) {
+        const int j = e / NIA + (e % NIA) * p.kA;  // NIA is a power of two
+        int32_t oa = (int32_t)kRcTabOOB, ob = 0;
+        if (j < p.C) col_offsets(p, (uint32_t)j, oa, ob);
+        cS[e] = oa;
+      }
+      for (int e = (int)tid; e < p.kB * NIB; e += kRcThreads) {
+        const int i = e / NIB + (e % NIB) * p.kB;
+        int32_t oa = 0, ob = (int32_t)kRcTabOOB;
+        if (i < p.R) row_offsets(p, (uint32_t)i, oa, ob);
+        rS[e] = ob;
+      }
+      int32_t oa = 0, ob = 0;
+      fa_row = (int32_t)kRcTabOOB;
+      if (ai < (uint32_t)p.R) row_offsets(p, ai, fa_row, ob);
+      fb_col = (int32_t)kRcTabOOB;
+      if (bj < (uint32_t)p.C) col_offsets(p, bj, oa, fb_col);
+    }
     __syncthreads();
   }
 
@@ -143,6 +219,8 @@ struct RCWorker {
     }
     c.bA = uniform64(c.bA);
     c.bB = uniform64(c.bB);
+    c.fast = FAST && p.fast && !lastA && !lastB && !empty && !pad;
+    if (c.fast) return c;
     c.rows = lastA ? (uint32_t)p.Rl : (uint32_t)p.R;
     c.cols = lastB ? (uint32_t)p.Cl : (uint32_t)p.C;
     const FastDiv dr = lastA ? p.dRl : p.dR;
@@ -151,21 +229,22 @@ struct RCWorker {
     return c;
   }
 
-  // Offset tables copied from host-computed ones (grouped launches:
-  // rowA[R], rowB[R], colA[C], colB[C] in `tab`).  The caller has passed a
-  // barrier since the previous tile's last LDS read.
+  // Offset tables copied from host-computed ones (grouped launches: the
+  // p.ntab entries of the LDS layout, rc_fill_tables).  The caller has
+  // passed a barrier since the previous tile's last LDS read.
   __device__ __forceinline__ void setup_tables(const RCParams& p, const int32_t* __restrict__ tab,
                                                unsigned char* lds_raw, uint32_t tid_) {
     tid = tid_;
     data = reinterpret_cast<L*>(lds_raw);
-    int32_t* t = reinterpret_cast<int32_t*>(lds_raw + (size_t)p.nbuf * (size_t)p.lds_data_elems * sizeof(L));
-    const int n = 2 * p.R + 2 * p.C;
-    for (int i = (int)tid; i < n; i += kRcThreads) t[i] = tab[i];
-    rowA = t;
-    rowB = t + p.R;
-    colA = t + 2 * p.R;
-    colB = t + 2 * p.R + p.C;
+    int32_t* t = tables(p, lds_raw);
+    for (int i = (int)tid; i < p.ntab; i += kRcThreads) t[i] = tab[i];
+    uint32_t ai = 0, bj = 0;
+    bind_tables(p, t, ai, bj);
     __syncthreads();
+    if (FAST && p.fast) {
+      fa_row = t[ai];
+      fb_col = colB[bj];
+    }
   }
 
   // Tile from a host-computed record (grouped launches): origin offsets and
@@ -177,6 +256,8 @@ struct RCWorker {
     c.bB = uniform64(rec.b);
     const uint32_t f = uniform(rec.flags);
     const bool lastA = (f & 1u) != 0, lastB = (f & 2u) != 0;
+    c.fast = FAST && p.fast && !lastA && !lastB && !empty;
+    if (c.fast) return c;
     c.rows = lastA ? (uint32_t)p.Rl : (uint32_t)p.R;
     c.cols = lastB ? (uint32_t)p.Cl : (uint32_t)p.C;
     const FastDiv dr = lastA ? p.dRl : p.dR;
@@ -222,12 +303,35 @@ struct RCWorker {
   }
   __device__ __forceinline__ void load(const RCParams& p, const Tile& c, const TA* __restrict__ A,
                                        TA (&va)[NIA][VA]) const {
-    if (p.rowA_dense) load_impl<true>(p, c, A, va);
-    else load_impl<false>(p, c, A, va);
+    if (FAST && c.fast) {
+      const BufRes r = make_res(A + c.bA);
+#pragma unroll
+      for (int it = 0; it < NIA; ++it) bload<TA, VA>(va[it], r, (uint32_t)fa_row + (uint32_t)fa_col[it]);
+    } else if (p.rowA_dense) {
+      load_impl<true>(p, c, A, va);
+    } else {
+      load_impl<false>(p, c, A, va);
+    }
   }
 
   __device__ __forceinline__ void stage(const RCParams& p, const Tile& c, const TA (&va)[NIA][VA],
                                         typename OP::AL alpha) const {
+    if (FAST && c.fast) {
+      // a slot past column C lands at or beyond RP * C: clamped to the spare
+      // slot, as are padding rows (fa_step 0)
+      const uint32_t spare = (uint32_t)p.lds_spare;
+#pragma unroll
+      for (int it = 0; it < NIA; ++it) {
+        const uint32_t li = min(fa_lds + (uint32_t)it * fa_step, spare);
+#pragma unroll
+        for (int v = 0; v < VA; ++v) {
+          const L x = OP::prod(alpha, va[it][v]);
+          if constexpr (FP8) amax = fmaxf(amax, fabsf(x));
+          data[li + v] = x;
+        }
+      }
+      return;
+    }
     const uint32_t RP = (uint32_t)p.RP;
     const uint32_t dli = c.adi + c.adj * RP, wrap = RP - c.rows;
     uint32_t i = c.ai, li = c.ai + c.aj * RP;
@@ -275,12 +379,21 @@ struct RCWorker {
     }
   }
 
+  // The lane-fixed write walk: slot it is row fb_row's it-th entry of this
+  // lane's column (offset kRcTabOOB past R or C).
+  template <class F>
+  __device__ __forceinline__ void walk_fast(F&& f) const {
+#pragma unroll
+    for (int it = 0; it < NIB; ++it) f(it, (uint32_t)fb_row[it] + (uint32_t)fb_col, fb_lds + (uint32_t)it * fb_step);
+  }
+
   // Old values of B for the beta epilogue.
   __device__ __forceinline__ void load_b(const RCParams& p, const Tile& c, const TB* __restrict__ B,
                                          TB (&ob)[NIB][VB]) const {
     const BufRes r = make_res(B + c.bB);
     auto f = [&](int it, uint32_t off, uint32_t) { bload<TB, VB>(ob[it], r, off); };
-    if (p.colB_dense) walk_b<true>(p, c, f);
+    if (FAST && c.fast) walk_fast(f);
+    else if (p.colB_dense) walk_b<true>(p, c, f);
     else walk_b<false>(p, c, f);
   }
 
@@ -306,7 +419,8 @@ struct RCWorker {
       }
       bstore<TB, VB, NT>(r, off, o);
     };
-    if (p.colB_dense) walk_b<true>(p, c, f);
+    if (FAST && c.fast) walk_fast(f);
+    else if (p.colB_dense) walk_b<true>(p, c, f);
     else walk_b<false>(p, c, f);
   }
 
@@ -387,13 +501,13 @@ __device__ __forceinline__ void rc_run_table(RCWorker<OP, VA, VB, BETA, NT>& w, 
 
 // The parameters are the kernel's first argument, so they sit at the start
 // of the kernel-argument segment.
-template <class OP, int VA, int VB, bool BETA, bool NT>
+template <class OP, int VA, int VB, bool BETA, bool NT, bool FAST>
 __global__ void __launch_bounds__(kRcThreads) transpose_rc_kernel(const RCParams p, const typename OP::TA* __restrict__ A,
                                                                 typename OP::TB* __restrict__ B, double alpha,
                                                                 double beta) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   KParams* kp = (KParams*)__builtin_amdgcn_kernarg_segment_ptr();
-  RCWorker<OP, VA, VB, BETA, NT> w;
+  RCWorker<OP, VA, VB, BETA, NT, FAST> w;
   w.setup(p, lds_raw, threadIdx.x);
   const TileRange r = tile_range(p.ntiles, blockIdx.x, gridDim.x, p.grid_order);
   // Work-group-uniform by construction; readfirstlane keeps them (and all
@@ -406,7 +520,7 @@ __global__ void __launch_bounds__(kRcThreads) transpose_rc_kernel(const RCParams
 // * scale)), amax = max |fl32(alpha*a)| (one atomic per work-group).  The
 // scale comes from *scale_in (delayed scaling), from fp8_max / *amax_in
 // (current scaling, amax_in from a previous pass), or from the host value.
-template <class OP, int VA, int VB>
+template <class OP, int VA, int VB, bool FAST>
 __global__ void __launch_bounds__(kRcThreads)
     cast_fp8_rc_kernel(const RCParams p, const typename OP::TA* __restrict__ A, typename OP::TB* __restrict__ B,
                        float alpha, const float* __restrict__ scale_in, const float* __restrict__ amax_in,
@@ -422,7 +536,7 @@ __global__ void __launch_bounds__(kRcThreads)
     scale = (m > 0.0f && m <= 3.402823466e38f) ? fp8_max / m : 1.0f;
   }
   if (scale_inv_out && blockIdx.x == 0 && threadIdx.x == 0) *scale_inv_out = 1.0f / scale;
-  RCWorker<OP, VA, VB, false, false> w;
+  RCWorker<OP, VA, VB, false, false, FAST> w;
   w.setup(p, lds_raw, threadIdx.x);
   const TileRange r = tile_range(p.ntiles, blockIdx.x, gridDim.x, p.grid_order);
   w.run(p, kp, A, B, uniform(r.first), uniform(r.end), uniform(r.step), alpha, scale);

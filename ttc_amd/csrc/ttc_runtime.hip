@@ -125,11 +125,13 @@ ttc_status_t launch_plan(const ttc_plan_s* pl, const void* A, void* B, double al
                ? TTC_SUCCESS
                : TTC_ERROR_HIP;
   }
-  const unsigned grid = grid_size(use->p.ntiles, pl->wg_per_cu, e->func, use->lds_bytes,
+  const bool fast = use->family == kFamRC && use->rc.fast;
+  const unsigned grid = grid_size(use->p.ntiles, pl->wg_per_cu, fast ? e->func_fast : e->func, use->lds_bytes,
                                   use->family == kFamRC ? kRcThreads : kThreads);
-  const hipError_t err = use->family == kFamRC
-                             ? e->launch_rc(use->rc, A, B, alpha, beta, grid, (size_t)use->lds_bytes, s)
-                             : e->launch(use->p, A, B, alpha, beta, grid, (size_t)use->lds_bytes, s);
+  const hipError_t err =
+      use->family == kFamRC
+          ? (fast ? e->launch_rc_fast : e->launch_rc)(use->rc, A, B, alpha, beta, grid, (size_t)use->lds_bytes, s)
+          : e->launch(use->p, A, B, alpha, beta, grid, (size_t)use->lds_bytes, s);
   return err == hipSuccess ? TTC_SUCCESS : TTC_ERROR_HIP;
 }
 
@@ -231,9 +233,12 @@ extern "C" ttc_status_t ttc_execute_fp8(ttc_plan_t pl, const void* A, void* B, d
                : TTC_ERROR_HIP;
   }
   if (use->family == kFamRC) {
-    const unsigned grid = grid_size(use->p.ntiles, pl->wg_per_cu, e->func, use->lds_bytes, kRcThreads);
-    return e->launch_fp8_rc(use->rc, A, B, al, d_scale_in, amax_in, fp8_max, scale, amax_out, d_scale_inv_out, grid,
-                            (size_t)use->lds_bytes, s) == hipSuccess
+    const bool fast = use->rc.fast != 0;
+    const unsigned grid = grid_size(use->p.ntiles, pl->wg_per_cu, fast ? e->func_fast : e->func, use->lds_bytes,
+                                    kRcThreads);
+    return (fast ? e->launch_fp8_rc_fast : e->launch_fp8_rc)(use->rc, A, B, al, d_scale_in, amax_in, fp8_max, scale,
+                                                             amax_out, d_scale_inv_out, grid, (size_t)use->lds_bytes,
+                                                             s) == hipSuccess
                ? TTC_SUCCESS
                : TTC_ERROR_HIP;
   }

@@ -70,7 +70,7 @@ class PlanInfo(ctypes.Structure):
                 ("type_a", ctypes.c_int), ("type_b", ctypes.c_int), ("beta_is_zero", ctypes.c_int),
                 ("run_a", ctypes.c_int), ("run_b", ctypes.c_int), ("family", ctypes.c_int),
                 ("rows", ctypes.c_int), ("cols", ctypes.c_int), ("tile_order", ctypes.c_int),
-                ("lds_buffers", ctypes.c_int)]
+                ("lds_buffers", ctypes.c_int), ("lane_fixed", ctypes.c_int)]
 
     def as_dict(self):
         r = self.rank
@@ -83,7 +83,8 @@ class PlanInfo(ctypes.Structure):
                 "elements": self.elements, "bytes_algorithmic": self.bytes_algorithmic,
                 "type_a": self.type_a, "type_b": self.type_b, "beta_is_zero": self.beta_is_zero,
                 "run_a": self.run_a, "run_b": self.run_b, "family": self.family, "rows": self.rows,
-                "cols": self.cols, "tile_order": self.tile_order, "lds_buffers": self.lds_buffers}
+                "cols": self.cols, "tile_order": self.tile_order, "lds_buffers": self.lds_buffers,
+                "lane_fixed": self.lane_fixed}
 
 
 class BatchOp(ctypes.Structure):
