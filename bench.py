@@ -139,6 +139,8 @@ def verify_pass(torch, ops, stream, key):
         sel = [o for o in ops if o["bz"] == bz]
         samples = [sample_offsets(torch, o) for o in sel]
         old = [o["B"][ib].clone() for o, (_, ib) in zip(sel, samples)]
+        # the snapshots run on torch's current stream, the ops on `stream`
+        torch.cuda.synchronize()
         for o in sel:
             o[key].execute(o["A"], o["B"], o["alpha"], o["beta"], stream=stream)
         torch.cuda.synchronize()
