@@ -8,10 +8,11 @@ B[pi(i)] = alpha*A[i] + beta*B[pi(i)], inputs resident in HBM.  All beta = 0
 ops run first, then all beta = 1 ops, so consecutive ops never touch the same
 buffers and no op finds its data in the 256 MB last-level cache.
 
-Plans: by default each op is autotuned first (at most 12 candidates timed
-on its own buffers, outside the timed region), as TTC generates and measures
-variants before use (ttc/ttc.py:871-1022); the planner's untuned choice is
-timed in the same run and reported beside it.
+Plans: by default each op is autotuned first (at most 200 candidates, TTC's
+default number of implementations, timed on its own buffers outside the
+timed region), as TTC generates and measures variants before use
+(ttc/ttc.py:871-1022); the planner's untuned choice is timed in the same run
+and reported beside it.
 
 Metric: GB/s (10^9 B/s) of algorithmic traffic, N*sA + N*sB (+ N*sB when
 beta != 0) (ttc/transposeGenerator.py:665-668), over the whole timed region,
@@ -638,9 +639,10 @@ def main():
     ap.add_argument("--no-workloads", dest="workloads", action="store_false", default=True,
                     help="skip the mixed-precision / complex and fp8 cast-transpose throughput leg")
     ap.add_argument("--calibrate-copy", action="store_true", help="run only the plain copy used to scale counters")
-    ap.add_argument("--tune", type=int, default=12,
+    ap.add_argument("--tune", type=int, default=200,
                     help="autotune each op over at most N candidates before timing, as TTC generates the fastest "
-                         "variant before use (0: the planner's choice only)")
+                         "variant before use; 200 is TTC's own default (ttc/ttc.py:1160); 0: the planner's "
+                         "choice only")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only; --gpus %d is not supported (a transposition shards as "
@@ -735,7 +737,7 @@ def main():
     if args.workloads:
         try:
             out["workloads"] = {"plain_c1": run_c1(torch, rt, wl, dev, stream)}
-            out["workloads"].update(run_workloads(torch, rt, wl, dev, stream, tune=min(args.tune, 8)))
+            out["workloads"].update(run_workloads(torch, rt, wl, dev, stream, tune=min(args.tune, 32)))
         except Exception as e:  # reported, not hidden
             out.setdefault("workloads", {})["error"] = str(e)
     out["cpu_baseline"] = None if args.no_cpu else cpu_baseline(args.cpu_seconds)
