@@ -175,6 +175,13 @@ def test_lane_fixed_walk(gpu):
         info = check(gpu, tA, tB, perm, size, 0.5, 1.0, lda, ldb, seed=i, config=dict(family=2, tile=tile))
         assert info["lane_fixed"] == 1, info
     assert n >= 26
+    # grouped B order: 65 tiles along rows in groups of 8 / 16 leave padding
+    # ids, which the walk processes (B read and written) as empty tiles
+    for order, ids in ((3, 2376), (4, 2640)):
+        for beta in (0.0, 0.75):
+            cfg = dict(family=2, tile=[62, 128], vec_a=2, vec_b=2, tile_order=order)
+            info = check(gpu, BF16, BF16, [1, 0], [4000, 4100], 1.5, beta, seed=order, config=cfg)
+            assert info["lane_fixed"] == 1 and info["tiles"] == ids > 65 * 33, info
 
 
 def test_explicit_row_column_tiles(gpu):
