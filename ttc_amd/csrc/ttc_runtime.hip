@@ -157,9 +157,9 @@ static hipError_t launch_amax(const Geometry& g, const void* A, float alpha, flo
   if (!q.dense && N >= (1ull << 31)) return hipErrorInvalidValue;
   int dev = 0;
   (void)hipGetDevice(&dev);
-  const uint64_t per = (uint64_t)kThreads * (q.dense ? VEC : 1);
+  const uint64_t per = (uint64_t)kThreads * (q.dense ? VEC * kAmaxUnroll : 1);
   uint64_t blocks = (N + per - 1) / per;
-  const uint64_t cap = (uint64_t)device_cus(dev) * 8;
+  const uint64_t cap = (uint64_t)device_cus(dev) * (q.dense ? 2 : 8);
   if (blocks > cap) blocks = cap;
   if (blocks < 1) blocks = 1;
   const TA* a = static_cast<const TA*>(A);
