@@ -516,12 +516,14 @@ __device__ __forceinline__ TileRange tile_range(uint32_t ntiles, uint32_t wg, ui
 }
 
 // Block-wide max of a non-negative float, then one atomicMax on its bits
-// (non-negative IEEE floats order like their unsigned bit patterns).  The
-// value only grows during a launch, so a block whose max does not exceed the
-// one already there skips its atomic: the work-groups of a launch end
-// together, and their same-address atomics would queue at one L2 channel.
+// (non-negative IEEE floats order like their unsigned bit patterns); NT is
+// the work-group size.  The value only grows during a launch, so a block
+// whose max does not exceed the one already there skips its atomic: the
+// work-groups of a launch end together, and their same-address atomics
+// would queue at one L2 channel.
+template <int NT = kThreads>
 __device__ __forceinline__ void block_amax_commit(float v, float* out) {
-  __shared__ float red[kThreads / 64];
+  __shared__ float red[NT / 64];
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -530,7 +532,7 @@ __device__ __forceinline__ void block_amax_commit(float v, float* out) {
   if (threadIdx.x == 0) {
     float m = red[0];
 #pragma unroll
-    for (int w = 1; w < kThreads / 64; ++w) m = fmaxf(m, red[w]);
+    for (int w = 1; w < NT / 64; ++w) m = fmaxf(m, red[w]);
     unsigned int* const o = reinterpret_cast<unsigned int*>(out);
     const unsigned int bits = __float_as_uint(m);
     if (bits > __hip_atomic_load(o, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(o, bits);

@@ -540,7 +540,7 @@ __global__ void __launch_bounds__(kRcThreads)
   w.setup(p, lds_raw, threadIdx.x);
   const TileRange r = tile_range(p.ntiles, blockIdx.x, gridDim.x, p.grid_order);
   w.run(p, kp, A, B, uniform(r.first), uniform(r.end), uniform(r.step), alpha, scale);
-  if (amax_out) block_amax_commit(w.amax, amax_out);
+  if (amax_out) block_amax_commit<kRcThreads>(w.amax, amax_out);
 }
 
 }  // namespace ttc
