@@ -534,7 +534,7 @@ def run_workloads(torch, rt, wl, dev, stream, reps=5, tune=0):
     candidates) when it times faster than the planner's, whose rate is
     reported beside it.  GB/s counts
     algorithmic bytes (fp8: N*s_A + N; the current-scaling amax pass re-reads
-    A and is not counted)."""
+    A, which only current_GBps_both_passes counts)."""
     import math
 
     def timed(fn, n=None):
@@ -595,6 +595,8 @@ def run_workloads(torch, rt, wl, dev, stream, reps=5, tune=0):
                                                           scale_inv_out=sinv, mode=mode, stream=stream), 20)
                     res[mname + "_ms"] = round(ms, 4)
                     res[mname + "_GBps"] = round(nb / (ms * 1e-3) / 1e9, 1)
+                # both passes of current scaling: the amax pass reads A once more
+                res["current_GBps_both_passes"] = round((nb + n * rt.DTYPE_BYTES[ta]) / (res["current_ms"] * 1e-3) / 1e9, 1)
                 res["frac_delayed"] = round(res["delayed_GBps"] / HBM_PEAK_GBS, 4)
                 out["fp8_%s_%s_%s" % (an, bn, "x".join(str(x) for x in shape["size"]))] = res
                 del Bs, plan
