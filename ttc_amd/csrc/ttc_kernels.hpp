@@ -515,6 +515,13 @@ __device__ __forceinline__ TileRange tile_range(uint32_t ntiles, uint32_t wg, ui
   return r;
 }
 
+// Work-groups of the row x column walk with at least this many tiles step
+// their wave priority down as they advance: those dispatched later onto a CU
+// lose issue arbitration to the older ones beside them, and the falling
+// priority lets them catch up.  With fewer tiles per work-group the schedule
+// measured slower, so it stays off.
+constexpr uint32_t kPrioMinRounds = 8;
+
 // Block-wide max of a non-negative float, then one atomicMax on its bits
 // (non-negative IEEE floats order like their unsigned bit patterns); NT is
 // the work-group size.  The value only grows during a launch, so a block

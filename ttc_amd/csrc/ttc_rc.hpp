@@ -444,16 +444,19 @@ struct RCWorker {
     L* const base = data;
     const uint32_t alt = p.nbuf == 2 ? (uint32_t)p.lds_data_elems : 0u;
     uint32_t off = 0;
-    // Wave priority falls as the work-group works through its tiles, so that
-    // a work-group behind its neighbours on the CU (dispatched later: older
-    // waves win arbitration) keeps the higher priority.
+    // Wave priority falls per quarter of the work-group's tiles, so that a
+    // work-group behind its neighbours on the CU (dispatched later: older
+    // waves win arbitration) keeps the higher priority (kPrioMinRounds).
     const uint32_t total = (end - first + step - 1) / step, q1 = total / 4, q2 = total / 2, q3 = total - total / 4;
+    const bool sched = total >= kPrioMinRounds;
     uint32_t k = 0;
     for (uint32_t t = first; t < end; t += step, ++k) {
-      if (k == 0) __builtin_amdgcn_s_setprio(3);
-      else if (k == q1) __builtin_amdgcn_s_setprio(2);
-      else if (k == q2) __builtin_amdgcn_s_setprio(1);
-      else if (k == q3) __builtin_amdgcn_s_setprio(0);
+      if (sched) {
+        if (k == 0) __builtin_amdgcn_s_setprio(3);
+        else if (k == q1) __builtin_amdgcn_s_setprio(2);
+        else if (k == q2) __builtin_amdgcn_s_setprio(1);
+        else if (k == q3) __builtin_amdgcn_s_setprio(0);
+      }
       data = base + off;
       stage(p, c, va, alpha);
       __syncthreads();

@@ -40,17 +40,7 @@ __global__ void __launch_bounds__(kThreads)
   char* b = reinterpret_cast<char*>(B);
   const uint32_t stride = gridDim.x * (uint32_t)kThreads;  // the host keeps total + U * stride < 2^32
   float amax = 0.0f;
-  // Wave priority steps down per quarter of the work-group's rounds (as in
-  // the row x column walk: work-groups dispatched later onto a CU catch up).
-  const uint32_t first = blockIdx.x * (uint32_t)kThreads, rstep = U * stride;
-  const uint32_t rounds = first < p.total ? (p.total - first + rstep - 1) / rstep : 0u;
-  const uint32_t q1 = rounds / 4, q2 = rounds / 2, q3 = rounds - rounds / 4;
-  uint32_t k = 0;
-  for (uint32_t g0 = first + threadIdx.x; g0 < p.total; g0 += rstep, ++k) {
-    if (k == 0) __builtin_amdgcn_s_setprio(3);
-    else if (k == q1) __builtin_amdgcn_s_setprio(2);
-    else if (k == q2) __builtin_amdgcn_s_setprio(1);
-    else if (k == q3) __builtin_amdgcn_s_setprio(0);
+  for (uint32_t g0 = blockIdx.x * (uint32_t)kThreads + threadIdx.x; g0 < p.total; g0 += U * stride) {
     int64_t ob[U];
     bool ok[U];
     TA va[U][V];
