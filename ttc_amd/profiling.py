@@ -10,11 +10,13 @@
       per-kernel average of every counter of a `rocprofv3 --pmc` run
       (for example SQ_LDS_BANK_CONFLICT against SQ_ACTIVE_INST_LDS).
   python -m ttc_amd.profiling traffic --fetch DIR --write DIR
-         --calib-fetch DIR --calib-write DIR --calib-bytes N [--last N] [-o OUT]
+         --calib-fetch DIR --calib-write DIR --calib-bytes N [--last N]
+         [--kernel K --kernel-last N] [-o OUT]
       HBM bytes per launch of every kernel from two `rocprofv3 --pmc` runs
       (FETCH_SIZE, then WRITE_SIZE; KiB per dispatch), each scaled by what
       the same counter reads for a plain copy of N bytes (`bench.py
-      --calibrate-copy`).  Writes OUT.md and OUT.json.
+      --calibrate-copy`).  Each kernel's last --last dispatches, kernel K's
+      last --kernel-last (its roofline window).  Writes OUT.md and OUT.json.
 
 The counters read kilobytes of L2-to-memory traffic per dispatch; on gfx950
 FETCH_SIZE counts 128-byte read requests as 64 bytes, which the copy
@@ -101,7 +103,10 @@ def _copy_kernel(per_kernel):
     return max(names, key=lambda k: len(per_kernel[k]))
 
 
-def traffic(fetch_dir, write_dir, calib_fetch_dir, calib_write_dir, calib_bytes, last=0):
+def traffic(fetch_dir, write_dir, calib_fetch_dir, calib_write_dir, calib_bytes, last=0, kernel=None, kernel_last=0):
+    """Bytes per launch of each kernel, from counter runs scaled by the copy
+    calibration: each kernel's last `last` dispatches (0: all), and for
+    `kernel` its last `kernel_last` (the window its roofline line times)."""
     cf = counter_per_kernel(calib_fetch_dir, "FETCH_SIZE")
     cw = counter_per_kernel(calib_write_dir, "WRITE_SIZE")
     kf, kw = _copy_kernel(cf), _copy_kernel(cw)
@@ -111,9 +116,11 @@ def traffic(fetch_dir, write_dir, calib_fetch_dir, calib_write_dir, calib_bytes,
     w_scale = calib_bytes / (write_kib * 1024.0)
     fetch = counter_per_kernel(fetch_dir, "FETCH_SIZE")
     write = counter_per_kernel(write_dir, "WRITE_SIZE")
-    if last:  # each kernel's last dispatches only
-        fetch = {k: v[-last:] for k, v in fetch.items()}
-        write = {k: v[-last:] for k, v in write.items()}
+    def window(k, v):
+        n = kernel_last if (kernel and kernel_last and k == kernel) else last
+        return v[-n:] if n else v
+    fetch = {k: window(k, v) for k, v in fetch.items()}
+    write = {k: window(k, v) for k, v in write.items()}
     kernels = {}
     for k in sorted(set(fetch) & set(write)):
         if "ttc::" not in k:
@@ -146,6 +153,8 @@ def main(argv=None):
         t.add_argument(a, required=True)
     t.add_argument("--calib-bytes", type=int, required=True)
     t.add_argument("--last", type=int, default=0, help="use only each kernel's last N dispatches")
+    t.add_argument("--kernel", help="kernel summarised over its own window (--kernel-last)")
+    t.add_argument("--kernel-last", type=int, default=0, help="that kernel's last N dispatches")
     t.add_argument("-o", "--out", required=True, help="output path without extension")
     t.add_argument("--command", default="", help="the commands that produced the directories")
     args = ap.parse_args(argv)
@@ -190,8 +199,11 @@ def main(argv=None):
                 f.write(text)
         sys.stdout.write(text)
         return 0
-    res = traffic(args.fetch, args.write, args.calib_fetch, args.calib_write, args.calib_bytes, args.last)
+    res = traffic(args.fetch, args.write, args.calib_fetch, args.calib_write, args.calib_bytes, args.last,
+                  args.kernel, args.kernel_last)
     res["last_dispatches"] = args.last or None
+    if args.kernel and args.kernel_last:
+        res["kernel_window"] = {"kernel": args.kernel, "last_dispatches": args.kernel_last}
     if args.command:
         res["command"] = args.command
     with open(args.out + ".json", "w") as f:
@@ -203,7 +215,9 @@ def main(argv=None):
     lines += ["Calibration: a plain copy of %d bytes (`%s`) reads FETCH_SIZE = %.1f KiB and WRITE_SIZE = %.1f KiB;"
               " scale factors %.4f (fetch) and %.4f (write)." % (c["copy_bytes"], c["copy_kernel"], c["fetch_kib"],
                                                                  c["write_kib"], c["fetch_scale"], c["write_scale"]),
-              "", ("Each kernel's last %d dispatches." % args.last) if args.last else "All dispatches.",
+              "", (("Each kernel's last %d dispatches." % args.last) if args.last else "All dispatches.") + (
+                  (" `%s`: its last %d dispatches (the launches its roofline line times)." % (
+                      args.kernel, args.kernel_last)) if args.kernel and args.kernel_last else ""),
               "", "| kernel | launches | fetch MB/launch | write MB/launch | traffic MB/launch |",
               "|---|---:|---:|---:|---:|"]
     for k, v in sorted(res["kernels"].items(), key=lambda kv: -kv[1]["traffic_bytes"] * kv[1]["launches"]):
