@@ -1,8 +1,9 @@
 // CPU check of the row x column kernels' index arithmetic, against the
 // planner's own parameters (ttc_plan.cpp): for many geometries, type pairs,
 // vector widths and tiles it replays, lane by lane and slot by slot, the
-// lane-fixed walk of full tiles that ttc_rc.hpp runs (setup tables, read
-// walk, staging, write walk) and checks that
+// general walk's LDS positions for full and partial tiles (check_general, on
+// a sample) and the lane-fixed walk of full tiles that ttc_rc.hpp runs
+// (setup tables, read walk, staging, write walk), checking for the latter that
 //   - every LDS data index stays inside the data region and every table
 //     index inside the table region the launch allocates;
 //   - the read walk visits each element of the tile exactly once with its
@@ -60,6 +61,79 @@ static void tables(const RCParams& p, std::vector<long long>& rA, std::vector<lo
       cB[j] += d * p.sbB[m];
     }
   }
+}
+
+// The general walk (every tile of a launch without the lane-fixed walk,
+// partial and empty tiles of one with it): replays finish(), stage() and
+// walk_b() for a tile of rows x cols and checks that the staging positions
+// of the read walk (element (i, j) at i + j * RP) and the positions the
+// write walk reads agree element by element, stay inside the data region,
+// and that padding slots stage only at or past the spare slot.
+static bool check_general(const RCParams& p, int szA, int szB, int va, int vb, int rows, int cols) {
+  const int NIA = rc_elems(szA) / va, NIB = rc_elems(szB) / vb;
+  const uint32_t SA = (uint32_t)kRcThreads * va, SB = (uint32_t)kRcThreads * vb;
+  const uint32_t RP = (uint32_t)p.RP, spare = (uint32_t)p.lds_spare;
+  CHECK(rows <= p.R && cols <= p.C && rows % va == 0 && cols % vb == 0, "tile %d x %d", rows, cols);
+  const long long n = (long long)rows * cols;
+  std::vector<int> seen_r(n, 0), seen_w(n, 0);
+  for (uint32_t tid = 0; tid < (uint32_t)kRcThreads; ++tid) {
+    // finish()
+    const int32_t limA = (int32_t)n - (int32_t)(tid * va), limB = (int32_t)n - (int32_t)(tid * vb);
+    const uint32_t aj = tid * va / rows, ai = tid * va - aj * rows, adj = SA / rows, adi = SA - adj * rows;
+    const uint32_t bi = tid * vb / cols, bj = tid * vb - bi * cols, bdi = SB / cols, bdj = SB - bdi * cols;
+    // stage()
+    uint32_t i = ai, j = aj, li = ai + aj * RP;
+    const uint32_t dli = adi + adj * RP, wrap = RP - rows;
+    for (int it = 0; it < NIA; ++it) {
+      const bool ok = (int32_t)(it * SA) < limA;
+      const uint32_t a0 = ok ? li : spare;
+      for (int v = 0; v < va; ++v) {
+        const uint32_t a = a0 + (uint32_t)v;
+        CHECK(a < (uint32_t)p.lds_data_elems, "stage index %u of %d", a, p.lds_data_elems);
+        if (ok) {
+          CHECK(i + v < (uint32_t)rows && j < (uint32_t)cols, "slot (%u, %u) outside %d x %d", i + v, j, rows, cols);
+          CHECK(a == (i + v) + j * RP, "stage position of (%u, %u)", i + v, j);
+          CHECK(seen_r[(i + v) + (long long)j * rows]++ == 0, "staged twice (%u, %u)", i + v, j);
+        } else {
+          CHECK(a >= spare, "padding staged at %u", a);
+        }
+      }
+      i += adi;
+      j += adj;
+      li += dli;
+      if (i >= (uint32_t)rows) {
+        i -= rows;
+        ++j;
+        li += wrap;
+      }
+    }
+    // walk_b() and the reads of store()
+    const uint32_t wdli = bdj * RP + bdi, wwrap = 1u - cols * RP;
+    uint32_t wi = bi, wj = bj, wl = bi + bj * RP;
+    for (int it = 0; it < NIB; ++it) {
+      const bool ok = (int32_t)(it * SB) < limB;
+      const uint32_t base = ok ? wl : 0u;
+      for (int v = 0; v < vb; ++v) {
+        const uint32_t a = base + (uint32_t)v * RP;
+        CHECK(a < (uint32_t)p.lds_data_elems, "write-walk index %u of %d", a, p.lds_data_elems);
+        if (ok) {
+          CHECK(wi < (uint32_t)rows && wj + v < (uint32_t)cols, "slot (%u, %u) outside", wi, wj + v);
+          CHECK(a == wi + (wj + v) * RP, "write-walk position of (%u, %u)", wi, wj + v);
+          CHECK(seen_w[wi + (long long)(wj + v) * rows]++ == 0, "read twice (%u, %u)", wi, wj + v);
+        }
+      }
+      wj += bdj;
+      wi += bdi;
+      wl += wdli;
+      if (wj >= (uint32_t)cols) {
+        wj -= cols;
+        ++wi;
+        wl += wwrap;
+      }
+    }
+  }
+  for (long long e = 0; e < n; ++e) CHECK(seen_r[e] == 1 && seen_w[e] == 1, "element %lld missed", e);
+  return true;
 }
 
 static bool check_walk(const RCParams& p, int szA, int szB, int va, int vb, int lds_bytes, int lds_elem) {
@@ -173,7 +247,7 @@ int main(int argc, char** argv) {
   const int pairs[][2] = {{TTC_S, TTC_S},    {TTC_D, TTC_D},       {TTC_C, TTC_C},       {TTC_Z, TTC_Z},
                           {TTC_BF16, TTC_BF16}, {TTC_S, TTC_BF16}, {TTC_D, TTC_S},       {TTC_S, TTC_D},
                           {TTC_C, TTC_Z},    {TTC_BF16, TTC_E4M3}, {TTC_S, TTC_E5M2},    {TTC_F16, TTC_S}};
-  long long checked = 0, fast = 0;
+  long long checked = 0, fast = 0, general = 0;
   for (int c = 0; c < ncases && !g_fail; ++c) {
     const int r = 2 + (int)(rng() % 5);
     int perm[8], size[8], lda[8], ldb[8];
@@ -225,7 +299,14 @@ int main(int argc, char** argv) {
               if (build_rc_params(g, tile.data(), ta, tb, va, vb, 0, 2, p, lds, nbuf) != TTC_SUCCESS) continue;
               ++checked;
               fast += p.fast != 0;
-              if (!check_walk(p, dtype_bytes(ta), dtype_bytes(tb), va, vb, lds, lds_elem_bytes(ta, tb))) {
+              bool ok = check_walk(p, dtype_bytes(ta), dtype_bytes(tb), va, vb, lds, lds_elem_bytes(ta, tb));
+              if (checked % 8 == 0 || va * vb > 2) {  // a sample of the narrow launches
+                ++general;
+                for (int rows : {p.R, p.Rl})
+                  for (int cols : {p.C, p.Cl})
+                    ok = ok && check_general(p, dtype_bytes(ta), dtype_bytes(tb), va, vb, rows, cols);
+              }
+              if (!ok) {
                 std::fprintf(stderr, "case %d types %d->%d va %d vb %d tile", c, ta, tb, va, vb);
                 for (int k = 0; k < g.rank; ++k) std::fprintf(stderr, " %d", tile[k]);
                 std::fprintf(stderr, " R %d C %d nbuf %d\n", p.R, p.C, nbuf);
@@ -235,7 +316,7 @@ int main(int argc, char** argv) {
         }
     }
   }
-  std::printf("rc walk check: %lld launches, %lld with the lane-fixed walk: %s\n", checked, fast,
-              g_fail ? "FAIL" : "ok");
+  std::printf("rc walk check: %lld launches, %lld with the lane-fixed walk, %lld general walks replayed: %s\n",
+              checked, fast, general, g_fail ? "FAIL" : "ok");
   return g_fail;
 }

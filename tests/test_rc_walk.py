@@ -1,8 +1,10 @@
-"""CPU replay of the row x column kernels' lane-fixed walk against the
-planner's parameters: LDS and table bounds, exact coverage of every tile
-element with its true A and B offsets, and out-of-range offsets for every
-padding slot (tests/c/rc_walk_check.cpp, compiled here with g++ together
-with the planner source)."""
+"""CPU replay of the row x column kernels' walks against the planner's
+parameters (tests/c/rc_walk_check.cpp, compiled here with g++ together with
+the planner source): for the lane-fixed walk, LDS and table bounds, exact
+coverage of every tile element with its true A and B offsets, and
+out-of-range offsets for every padding slot; for the general walk on a
+sample of launches, full and partial tiles, that the staging and read
+positions agree element by element and stay in bounds."""
 import os
 import subprocess
 
