@@ -25,6 +25,7 @@
 using namespace ttc;
 
 static int g_fail = 0;
+static long long big_a = 0;  // lane-fixed launches whose A offsets pass 2^30 (no padding slots on A's side)
 #define CHECK(cond, ...)                                  \
   do {                                                    \
     if (!(cond)) {                                        \
@@ -150,10 +151,17 @@ static bool check_walk(const RCParams& p, int szA, int szB, int va, int vb, int 
   CHECK(p.lds_spare + 16 <= p.lds_data_elems, "spare %d data %d", p.lds_spare, p.lds_data_elems);
   std::vector<long long> rA, rB, cA, cB;
   tables(p, rA, rB, cA, cB);
-  for (long long x : rA) CHECK(x < kRcFastSpan, "rowA %lld", x);
-  for (long long x : cA) CHECK(x < kRcFastSpan, "colA %lld", x);
-  for (long long x : rB) CHECK(x < kRcFastSpan, "rowB %lld", x);
-  for (long long x : cB) CHECK(x < kRcFastSpan, "colB %lld", x);
+  // a side whose walk has padding slots keeps its entries below kRcFastSpan
+  // (sentinel sums stay out of range); entries of the other side are real
+  // offsets, bounded by the span check below
+  const long long capA = (long long)kRcThreads * rc_elems(szA), capB = (long long)kRcThreads * rc_elems(szB);
+  const bool padA = p.Rw > p.R || (long long)p.Rw * p.C < capA, padB = p.Cw > p.C || (long long)p.R * p.Cw < capB;
+  const long long limA = padA ? kRcFastSpan : 0x80000000ll - 64, limB = padB ? kRcFastSpan : 0x80000000ll - 64;
+  for (long long x : rA) CHECK(x < limA, "rowA %lld", x);
+  for (long long x : cA) CHECK(x < limA, "colA %lld", x);
+  for (long long x : rB) CHECK(x < limB, "rowB %lld", x);
+  for (long long x : cB) CHECK(x < limB, "colB %lld", x);
+  if (!padA) big_a += rA.back() + cA.back() >= kRcFastSpan;
   // LDS table contents (32-bit, as the kernel holds them)
   const long long oRB = p.Rw, oCA = oRB + p.R, oCB = oCA + p.C, oCS = oCB + p.Cw, oRS = oCS + (long long)p.kA * NIA;
   std::vector<uint32_t> tab(p.ntab, 0xdeadbeefu);
@@ -277,6 +285,9 @@ int main(int argc, char** argv) {
       lda[k] = size[k] + (pad ? (int)(rng() % 3) : 0);
       ldb[k] = size[perm[k]] + (pad ? (int)(rng() % 3) : 0);
     }
+    // one case in eight: a padded outer dimension of A large enough that
+    // tile spans pass 2^30 bytes (sub-tensor of a much larger array)
+    if (rng() % 8 == 0 && r >= 3) lda[r - 2] = size[r - 2] * (40 + (int)(rng() % 40));
     Geometry g;
     if (make_geometry(r, perm, size, lda, ldb, g) != TTC_SUCCESS || g.elements == 0 || g.rank < 2 || g.perm[0] == 0)
       continue;
@@ -316,7 +327,40 @@ int main(int argc, char** argv) {
         }
     }
   }
-  std::printf("rc walk check: %lld launches, %lld with the lane-fixed walk, %lld general walks replayed: %s\n",
-              checked, fast, general, g_fail ? "FAIL" : "ok");
+  // fixed cases whose lane-fixed walk runs with A offsets past 2^30 (no
+  // padding slots on A's side): fp32 -> bf16 and bf16 -> bf16 of a
+  // (256, 192, 128, 96) tensor, perm (3, 1, 0, 2), whose dimension 3 has a
+  // 25 MB (12.6 MB) stride in A
+  for (const int tb : {TTC_BF16}) {
+    for (const int ta : {TTC_S, TTC_BF16}) {
+      const int perm[4] = {3, 1, 0, 2}, size[4] = {256, 192, 128, 96};
+      const int lda[4] = {256, 192, 128, 96}, ldb[4] = {96, 192, 256, 128};
+      Geometry g;
+      if (make_geometry(4, perm, size, lda, ldb, g) != TTC_SUCCESS) return 1;
+      const int tiles[][4] = {{128, 1, 1, 64}, {256, 1, 1, 32}, {64, 1, 1, 64}};
+      for (const auto& t4 : tiles)
+        for (int nbuf = 1; nbuf <= 2 && !g_fail; ++nbuf) {
+          int tile[kMaxDim] = {1, 1, 1, 1, 1, 1, 1, 1};
+          for (int k = 0; k < 4; ++k) tile[k] = t4[k];
+          const int va = widest_vec_a(g, ta), vb = widest_vec_b(g, tb);
+          RCParams p;
+          int lds = 0;
+          if (build_rc_params(g, tile, ta, tb, va, vb, 0, 2, p, lds, nbuf) != TTC_SUCCESS) continue;
+          ++checked;
+          fast += p.fast != 0;
+          bool ok = check_walk(p, dtype_bytes(ta), dtype_bytes(tb), va, vb, lds, lds_elem_bytes(ta, tb));
+          for (int rows : {p.R, p.Rl})
+            for (int cols : {p.C, p.Cl})
+              ok = ok && check_general(p, dtype_bytes(ta), dtype_bytes(tb), va, vb, rows, cols);
+          if (!ok) {
+            std::fprintf(stderr, "fixed case types %d->%d tile %d %d %d %d nbuf %d\n", ta, tb, t4[0], t4[1], t4[2],
+                         t4[3], nbuf);
+            return 1;
+          }
+        }
+    }
+  }
+  std::printf("rc walk check: %lld launches, %lld with the lane-fixed walk, %lld general walks replayed, "
+              "%lld lane-fixed with A offsets past 2^30: %s\n", checked, fast, general, big_a, g_fail ? "FAIL" : "ok");
   return g_fail;
 }

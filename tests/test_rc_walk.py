@@ -4,7 +4,8 @@ the planner source): for the lane-fixed walk, LDS and table bounds, exact
 coverage of every tile element with its true A and B offsets, and
 out-of-range offsets for every padding slot; for the general walk on a
 sample of launches, full and partial tiles, that the staging and read
-positions agree element by element and stay in bounds."""
+positions agree element by element and stay in bounds; and lane-fixed launches
+whose A offsets pass 2^30 (no padding slots on A's side)."""
 import os
 import subprocess
 
@@ -25,5 +26,5 @@ def test_lane_fixed_walk_index_arithmetic(tmp_path):
     assert r.returncode == 0, r.stdout
     line = r.stdout.strip().splitlines()[-1]
     assert line.endswith(": ok"), line
-    checked, fast = [int(w) for w in line.replace(",", " ").split() if w.isdigit()][:2]
-    assert checked > 10000 and fast > checked // 40, line
+    checked, fast, general, big = [int(w) for w in line.replace(",", " ").split() if w.isdigit()][:4]
+    assert checked > 10000 and fast > checked // 40 and general > checked // 10 and big > 0, line

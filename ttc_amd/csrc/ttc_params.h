@@ -130,9 +130,11 @@ struct RCParams {
 // tile data: rowA[Rw] rowB[R] colA[C] colB[Cw], then (fast walk only)
 // colS[kA * NIA] and rowS[kB * NIB], the read walk's column offsets and the
 // write walk's row offsets in slot order (colS[jl * NIA + it] = colA[jl + it
-// * kA]); entries past R or C hold kRcTabOOB.  A table entry is below 2^30 -
-// 64, so the sum of two entries is either a real offset (< 2^31) or, if
-// either is kRcTabOOB, in [2^31, 2^32): past the buffer range 2^31.
+// * kA]); entries past R or C hold kRcTabOOB.  On a side whose walk has such
+// slots a table entry is below 2^30 - 64, so the sum of two entries is
+// either a real offset (< 2^31) or, if either is kRcTabOOB, in [2^31, 2^32):
+// past the buffer range 2^31.  A side without them holds real offsets only,
+// whose sums stay below 2^31 (the tile's span).
 constexpr uint32_t kRcTabOOB = 0xC0000000u;
 constexpr long long kRcFastSpan = (1ll << 30) - 64;
 

@@ -609,7 +609,10 @@ ttc_status_t build_rc_params(const Geometry& g, const int* tile, int ta, int tb,
   if (nbuf < 1 || nbuf > 2) return TTC_ERROR_INVALID_VALUE;
   // Lane-fixed walk of full tiles: the walks' row / column extents rounded
   // up to powers of two must divide the walk steps and still cover the tile
-  // within each side's slots, and table entries must stay below kRcFastSpan.
+  // within each side's slots, and a side whose walk has padding slots (its
+  // tables then hold kRcTabOOB) must keep its table entries below
+  // kRcFastSpan; without padding slots the buffer range bounds them (a
+  // 64-column fp32 tile of a tensor whose column stride is 25 MB spans 1.6 GB).
   // Padding lanes cost memory instructions: at most 1/16 of either walk may
   // be padding (measured: rows 192 -> 256 or 100 -> 128 lost 8-10%).  Its
   // longer setup pays off only over a few tiles per work-group: launches of
@@ -622,9 +625,10 @@ ttc_status_t build_rc_params(const Geometry& g, const int* tile, int ta, int tb,
   long long Rw = 1, Cw = 1;
   while (Rw < R) Rw <<= 1;
   while (Cw < C) Cw <<= 1;
+  const bool padA = Rw > R || Rw * C < capA, padB = Cw > C || R * Cw < capB;  // sentinels in a side's tables
   bool fast = Rw <= (long long)kRcThreads * va && Cw <= (long long)kRcThreads * vb && Rw * C <= capA &&
-              R * Cw <= capB && spanA * szA < kRcFastSpan && spanB * szB < kRcFastSpan && 16 * Rw <= 17 * R &&
-              16 * Cw <= 17 * C && ntiles >= kRcFastTiles && (szA == 2 || szB == 2);
+              R * Cw <= capB && (!padA || spanA * szA < kRcFastSpan) && (!padB || spanB * szB < kRcFastSpan) &&
+              16 * Rw <= 17 * R && 16 * Cw <= 17 * C && ntiles >= kRcFastTiles && (szA == 2 || szB == 2);
   const int le = lds_elem_bytes(ta, tb);
   long long RP = 0, data = 0, tab = 0;
   rc_layout(R, C, Rw, Cw, capA, capB, le, fast, RP, data, tab);
