@@ -115,6 +115,34 @@ def test_mixed_and_complex_full_size_sampled(gpu, code):
     assert np.isfinite(err_abs)
 
 
+def test_lane_fixed_walk_with_a_offsets_past_2_30(gpu):
+    """fp32 -> bf16 at the full mixed-workload size on [128, 1, 1, 64] tiles:
+    A's dimension 3 (25 MB stride) spans 1.6 GB per tile, so the lane-fixed
+    walk's A-side table entries pass 2^30; its A walk has no padding slots,
+    which is what lets it run.  Sampled outputs bit for bit."""
+    import torch
+    from ttc_amd import runtime as rt
+    n = int(np.prod(C3_SIZE))
+    gen = torch.Generator(device=gpu)
+    gen.manual_seed(31)
+    alpha, beta = 1.3, 0.7
+    A = _draw(torch, n, gpu, gen, torch.float32)
+    B = _draw(torch, n, gpu, gen, torch.bfloat16)
+    rng = np.random.default_rng(34)
+    offA, offB = _offsets(C3_PERM, C3_SIZE, rng, NSAMPLE)
+    tiA, tiB = torch.from_numpy(offA).to(gpu), torch.from_numpy(offB).to(gpu)
+    a_s, b_old = A[tiA].cpu(), B[tiB].cpu()
+    plan = rt.Plan(S, BF16, C3_PERM, C3_SIZE, beta_is_zero=False, config={"family": 2, "tile": [128, 1, 1, 64]})
+    info = plan.info()
+    assert info["lane_fixed"] == 1 and info["tile"][:4] == [128, 1, 1, 64], info
+    plan.execute(A, B, alpha, beta)
+    torch.cuda.synchronize()
+    got = B[tiB].cpu()
+    del A, B
+    want = _real_update(S, BF16, a_s.numpy(), b_old.view(torch.int16).numpy().view(np.uint16), alpha, beta, False)
+    assert np.array_equal(got.view(torch.int16).numpy().view(np.uint16), want)
+
+
 FP8_CASES = [([1, 0], [8192, 14336]), ([0, 2, 1, 3], [128, 4096, 32, 8])]
 
 
